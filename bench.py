@@ -1,0 +1,250 @@
+#!/usr/bin/env python3
+"""Benchmark of the gemmMfma path on one MI355X.
+
+Headline workload (one "step" = one GEMM launch): bf16 GEMM with fp32
+accumulate, column-major NN, M = N = K = 4096, alpha = 1, beta = 0, A and B
+i.i.d. normal(0, 1/sqrt(K)) from seed 1, inputs resident in HBM.  The JSON
+line reports TFLOP/s (2*M*N*K per launch) and the fraction of the bf16 MFMA
+peak, the roofline of the dominant kernel measured with HIP events on the
+launch stream, the CPU reference GEMM timed on the host cores beside it, and
+the other forms of the metric (fp16, NT/TN, beta = 1, fp8, bf16x3 vs fp32,
+grouped) under "extras".
+
+usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-extras] [--no-cpu]
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import math
+import os
+import sys
+import time
+
+PEAK_TFLOPS = {"bf16": 2500.0, "f16": 2500.0, "f8": 5000.0, "f32": 157.3, "f64": 78.6}
+HBM_GBPS = 8000.0
+
+
+def _args():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=200)
+    ap.add_argument("--warmup", type=int, default=20)
+    ap.add_argument("--no-extras", action="store_true")
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--only", default="", help="run only this extra (profiling)")
+    return ap.parse_args()
+
+
+def time_launches(fn, steps, warmup, torch):
+    """Average duration of fn() over `steps` back-to-back launches, HIP events
+    on the stream the kernels are launched on (torch's current stream)."""
+    for _ in range(warmup):
+        fn()
+    torch.cuda.synchronize()
+    s = torch.cuda.current_stream()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record(s)
+    for _ in range(steps):
+        fn()
+    e1.record(s)
+    torch.cuda.synchronize()
+    return e0.elapsed_time(e1) / steps  # ms
+
+
+def main():
+    a = _args()
+    if a.gpus != 1:
+        print(f"bench.py: this build measures one GPU; --gpus {a.gpus} is not supported "
+              "(the GEMM path is single-GPU; see DESIGN.md for how it would shard)", file=sys.stderr)
+        sys.exit(2)
+    import numpy as np
+    import torch
+
+    import tensilelite_amd as tl
+    from tensilelite_amd import _capi as C
+
+    dev = torch.device("cuda:0")
+    torch.cuda.set_device(dev)
+    M = N = K = 4096
+    flop = 2.0 * M * N * K
+    g = torch.Generator(device="cpu").manual_seed(1)
+    A = (torch.randn(M * K, generator=g) / math.sqrt(K)).to(torch.bfloat16).to(dev)
+    B = (torch.randn(K * N, generator=g) / math.sqrt(K)).to(torch.bfloat16).to(dev)
+    D = torch.empty(M * N, dtype=torch.bfloat16, device=dev)
+    prob = tl.make_problem("NN", C.BF16, M, N, K, M, K, M, M)
+    tile = tl.query_tile(prob)
+    fn = lambda: tl.run(prob, A, B, None, D)
+
+    # timed region: barrier + sync on both sides (one rank)
+    for _ in range(a.warmup):
+        fn()
+    torch.cuda.synchronize()
+    s = torch.cuda.current_stream()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    t0 = time.perf_counter()
+    e0.record(s)
+    for _ in range(a.steps):
+        fn()
+    e1.record(s)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    ms = e0.elapsed_time(e1) / a.steps
+    tflops = flop / (ms * 1e-3) / 1e12
+
+    out = {
+        "metric": "TFLOP/s & % MFMA peak, fp16/bf16/fp8 4096^3; bf16x3 vs fp32 at equal error",
+        "value": round(tflops, 2),
+        "unit": "TFLOP/s",
+        "n_gpus": 1,
+        "steps": a.steps,
+        "warmup": a.warmup,
+        "ms_per_step": round(ms, 5),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "bf16",
+        "data": "synthetic: A,B ~ normal(0,1/sqrt(K)) seed 1 (torch CPU generator), resident in HBM",
+        "config": {"workload": "gemm bf16 NN M=N=K=4096 alpha=1 beta=0, fp32 accumulate, bf16 out",
+                   "M": M, "N": N, "K": K, "layout": "NN", "tile": [tile.mt_m, tile.mt_n, tile.depth_u],
+                   "parallelism": "single GPU"},
+        "pct_of_peak": round(100.0 * tflops / PEAK_TFLOPS["bf16"], 2),
+        "roofline": {"bound": "mfma", "achieved": round(tflops, 2), "peak": PEAK_TFLOPS["bf16"],
+                     "unit": "TFLOP/s", "frac": round(tflops / PEAK_TFLOPS["bf16"], 4),
+                     "traffic": None, "kernel": "ffgemm::gemm_kernel<bf16,NN,%dx%d>" % (tile.mt_m, tile.mt_n),
+                     "algorithmic_flop_per_launch": flop, "wall_s": round(wall, 4)},
+    }
+
+    extras = {}
+    if not a.no_extras:
+        extras = run_extras(a, torch, np, tl, C, dev)
+    out["extras"] = extras
+
+    if not a.no_cpu:
+        out["cpu_baseline"] = cpu_baseline(np, ms)
+    print(json.dumps(out))
+
+
+def run_extras(a, torch, np, tl, C, dev):
+    ex = {}
+    steps = max(10, a.steps // 4)
+    M = N = K = 4096
+    flop = 2.0 * M * N * K
+    g = torch.Generator(device="cpu").manual_seed(1)
+    only = a.only
+
+    def want(name):
+        return not only or only == name
+
+    # fp16 / bf16 across layouts and beta
+    for dt, tdt in (("bf16", torch.bfloat16), ("f16", torch.float16)):
+        A = (torch.randn(M * K, generator=g) / math.sqrt(K)).to(tdt).to(dev)
+        B = (torch.randn(K * N, generator=g) / math.sqrt(K)).to(tdt).to(dev)
+        C0 = (torch.rand(M * N, generator=g) * 2 - 1).to(tdt).to(dev)
+        D = torch.empty(M * N, dtype=tdt, device=dev)
+        for layout in ("NN", "NT", "TN"):
+            for beta in (0.0, 1.0):
+                name = f"{dt}_{layout}_beta{int(beta)}_4096"
+                if not want(name):
+                    continue
+                p = tl.make_problem(layout, tdt, M, N, K, M if layout[0] == "N" else K,
+                                    K if layout[1] == "N" else N, M, M, 1.0, beta)
+                ms = time_launches(lambda: tl.run(p, A, B, C0, D), steps, 5, torch)
+                tf = flop / (ms * 1e-3) / 1e12
+                ex[name] = {"tflops": round(tf, 1), "ms": round(ms, 4), "pct_peak": round(100 * tf / 2500.0, 1)}
+        del A, B, C0, D
+
+    # torch (hipBLASLt) bf16 at the same size, for context only
+    if want("torch_bf16_4096"):
+        x = torch.randn(M, K, dtype=torch.bfloat16, device=dev)
+        y = torch.randn(K, N, dtype=torch.bfloat16, device=dev)
+        ms = time_launches(lambda: torch.mm(x, y), steps, 5, torch)
+        ex["torch_bf16_4096_hipblaslt"] = {"tflops": round(flop / (ms * 1e-3) / 1e12, 1), "ms": round(ms, 4)}
+        del x, y
+
+    # fp8 e4m3 with per-tensor scales, 4096^3 and 8192^3
+    for S in (4096, 8192):
+        name = f"f8_TN_{S}"
+        if not want(name):
+            continue
+        x = torch.randn(S * S, device=dev)
+        y = torch.randn(S * S, device=dev)
+        qa, sa = tl.fp8.quantize(x, rows=S, k=S, ldx=S, trans_x=False)
+        qb, sb = tl.fp8.quantize(y, rows=S, k=S, ldx=S, trans_x=True)
+        D = torch.empty(S * S, dtype=torch.float32, device=dev)
+        p = tl.make_problem("TN", C.F8E4M3, S, S, S, S, S, S, S, scaleA=sa, scaleB=sb, dtype_out=C.F32)
+        ms = time_launches(lambda: tl.run(p, qa, qb, None, D), max(5, steps // 4), 3, torch)
+        tf = 2.0 * S ** 3 / (ms * 1e-3) / 1e12
+        ex[name] = {"tflops": round(tf, 1), "ms": round(ms, 4), "pct_peak": round(100 * tf / 5000.0, 1)}
+        del x, y, qa, qb, D
+
+    # precision: fp32 MFMA vs bf16x3 at 8192^3 (A, B ~ normal(0,1) seed 2)
+    S = 8192
+    if want("precision_8192"):
+        g2 = torch.Generator(device="cpu").manual_seed(2)
+        A = torch.randn(S * S, generator=g2).to(dev)
+        B = torch.randn(S * S, generator=g2).to(dev)
+        D = torch.empty(S * S, device=dev)
+        res = {}
+        for comp, name in ((C.COMPUTE_DEFAULT, "f32_mfma"), (C.COMPUTE_BF16X3, "bf16x3")):
+            p = tl.make_problem("NN", C.F32, S, S, S, S, S, S, S, compute=comp)
+            ms = time_launches(lambda: tl.run(p, A, B, None, D), 3, 1, torch)
+            tf = 2.0 * S ** 3 / (ms * 1e-3) / 1e12
+            # error vs the FP64 GPU oracle product on 256 sampled columns
+            cols = 256
+            Dref = torch.empty(S * cols, dtype=torch.float64, device=dev)
+            p64 = tl.make_problem("NN", C.F64, S, cols, S, S, S, S, S)
+            tl.run(p64, A.double(), B[: S * cols].double(), None, Dref)
+            torch.cuda.synchronize()
+            err = (torch.linalg.norm(D[: S * cols].double() - Dref) / torch.linalg.norm(Dref)).item()
+            res[name] = {"tflops": round(tf, 1), "ms": round(ms, 3), "rel_fro_err_vs_fp64": err}
+        res["bf16x3_over_f32"] = round(res["bf16x3"]["tflops"] / res["f32_mfma"]["tflops"], 2)
+        ex["precision_8192"] = res
+        del A, B, D
+
+    # grouped: 64 problems, one persistent launch, hipGraph replay
+    if want("grouped_64"):
+        from tensilelite_amd import grouped
+        ex["grouped_64"] = grouped.bench_config(dev, steps=max(10, steps // 4))
+    return ex
+
+
+def cpu_baseline(np, gpu_ms):
+    """The oracle's restatement of the reference CPU GEMM (src/gemmMfma.cpp:
+    325-346), fp32, k innermost, timed on this host: a bounded sample of the
+    headline workload (64 of the 4096 output columns of the 4096^3 NN GEMM)
+    on all cores, plus the reference's own CPU case (480x512x512) on 1 core."""
+    sys.path.insert(0, os.path.dirname(os.path.abspath(__file__)))
+    import oracle
+
+    cores = len(os.sched_getaffinity(0))
+    M = K = 4096
+    ncols = 64
+    rng = np.random.default_rng(1)
+    A = (rng.standard_normal(M * K) / 64.0).astype(np.float32)
+    B = (rng.standard_normal(K * ncols) / 64.0).astype(np.float32)
+    t0 = time.perf_counter()
+    oracle.gemm_f32("NN", M, ncols, K, 1.0, A, M, B, K, nthreads=cores)
+    dt = time.perf_counter() - t0
+    tf = 2.0 * M * ncols * K / dt / 1e12
+    # reference CPU case: fp32 NN 480x512x512, U[-1,1] seed 0, one core, median of 3
+    m, n, k = 480, 512, 512
+    r0 = np.random.default_rng(0)
+    a = r0.uniform(-1, 1, m * k).astype(np.float32)
+    b = r0.uniform(-1, 1, k * n).astype(np.float32)
+    ts = []
+    for _ in range(3):
+        t1 = time.perf_counter()
+        oracle.gemm_f32("NN", m, n, k, 1.0, a, m, b, k)
+        ts.append(time.perf_counter() - t1)
+    t1 = sorted(ts)[1]
+    return {"value": round(tf, 6), "unit": "TFLOP/s", "cores": cores, "kind": "port",
+            "sample": f"4096x{ncols}x4096 fp32 NN (64 of 4096 columns of the headline GEMM), "
+                      f"{dt:.2f} s on {cores} threads",
+            "gpu_speedup": round((2.0 * 4096 ** 3 / (gpu_ms * 1e-3) / 1e12) / tf, 1),
+            "config1_480x512x512_1core_s": round(t1, 4),
+            "config1_gflops_1core": round(2.0 * m * n * k / t1 / 1e9, 3)}
+
+
+if __name__ == "__main__":
+    main()

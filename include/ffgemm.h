@@ -1,0 +1,162 @@
+/*
+ * ffgemm -- MI355X (gfx950) GEMM for the TensileLite gemmMfma problem.
+ *
+ * C ABI of libffgemm.so.  Plain pointers and sizes only; every matrix pointer
+ * is a device pointer owned by the caller.  The library never allocates or
+ * frees caller buffers; it owns only its grouped-GEMM plans (created and
+ * destroyed explicitly).
+ *
+ * Problem (column-major, as the reference's gemmMfma problem):
+ *     D = alpha * op(A) * op(B) + beta * C
+ *     op(A) is M x K, op(B) is K x N, C and D are M x N with leading
+ *     dimensions ldc / ldd.  Layout names follow BLAS: the first letter is
+ *     op(A), the second op(B); "T" means the operand is stored transposed.
+ *     The reference's only layout is TN: A is lda x M (K contiguous), B is
+ *     ldb x N (K contiguous), C/D are ldc x N (M contiguous)
+ *     (src/gemmMfma.cpp:210-217, 334-342).
+ *
+ * Status codes mirror E_ReturnState (inc/ff_utils.h:44-50).  No call exits
+ * the process; the last error message is kept per thread
+ * (ffgemm_last_error()).
+ *
+ * Every entry point taking a stream is asynchronous and stream-ordered,
+ * performs no allocation and no host synchronisation, and may be captured
+ * into a hipGraph.
+ */
+#ifndef FFGEMM_H
+#define FFGEMM_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* E_ReturnState: SUCCESS, RTN_WARN, RTN_ERR, RTN_FATAL (inc/ff_utils.h:44-50). */
+typedef enum {
+    FFGEMM_SUCCESS = 0,
+    FFGEMM_WARN = 1,
+    FFGEMM_ERR = 2,   /* invalid or unsupported problem; nothing launched */
+    FFGEMM_FATAL = 3  /* HIP runtime failure */
+} ffgemm_status;
+
+typedef enum { FFGEMM_NN = 0, FFGEMM_NT = 1, FFGEMM_TN = 2, FFGEMM_TT = 3 } ffgemm_layout;
+
+/* 1..3 are the reference's -d codes (src/gemmMfma.cpp:55-57). */
+typedef enum {
+    FFGEMM_F32 = 1,
+    FFGEMM_F16 = 2,
+    FFGEMM_BF16 = 3,
+    FFGEMM_F8E4M3 = 4, /* OCP e4m3 (e4m3fn), TN only; see ffgemm_fp8_quantize */
+    FFGEMM_F64 = 5     /* FP64 MFMA product, the oracle for error reports */
+} ffgemm_dtype;
+
+typedef enum {
+    FFGEMM_COMPUTE_DEFAULT = 0, /* f16/bf16/f8: MFMA with fp32 accumulate; f32: fp32 MFMA */
+    FFGEMM_COMPUTE_BF16X3 = 1   /* f32 in/out: hi/lo bf16 split, three MFMA products */
+} ffgemm_compute;
+
+/* Output rounding for bf16 D. */
+typedef enum {
+    FFGEMM_ROUND_RNE = 0,     /* round to nearest even (default) */
+    FFGEMM_ROUND_TRUNCATE = 1 /* the reference's bf16 store (gemmMfmaKernelWriter.h:1661-1664) */
+} ffgemm_round;
+
+typedef struct ffgemm_problem {
+    int32_t layout;    /* ffgemm_layout */
+    int32_t dtype;     /* ffgemm_dtype of A and B */
+    int32_t dtype_out; /* dtype of C and D; 0 = same as dtype (f8 -> f32) */
+    int32_t compute;   /* ffgemm_compute */
+    int32_t round;     /* ffgemm_round (bf16 output only) */
+    int32_t reserved;
+    int64_t M, N, K, batch; /* batch 0 is treated as 1 */
+    int64_t lda, ldb, ldc, ldd;
+    int64_t bsA, bsB, bsC, bsD; /* batch strides in elements */
+    float alpha, beta;
+    /* FP8 only: device pointers to the fp32 per-tensor quantisation scales of
+     * A and B (q = x * scale); D is multiplied by 1/(scaleA*scaleB).
+     * NULL means 1.0. */
+    const float *scaleA;
+    const float *scaleB;
+} ffgemm_problem;
+
+/* Tuning record; the fields the reference's -r -s -x -y -z -u -f flags set
+ * (src/gemmMfmaKernelWriter.h:11-22) plus gfx950 knobs.  A zeroed record (or
+ * a NULL pointer) asks the library to pick. */
+typedef struct ffgemm_tile {
+    int32_t mt_m;     /* macro tile along M (64, 128, 256) */
+    int32_t mt_n;     /* macro tile along N (64, 128, 256) */
+    int32_t depth_u;  /* K per main-loop iteration (fixed by dtype: 128 bytes of K) */
+    int32_t waves_m;  /* waves along M (2) */
+    int32_t waves_n;  /* waves along N (2) */
+    int32_t mfma_mn;  /* MFMA M/N width (32) */
+    int32_t group_m;  /* tile-raster group height (0 = default) */
+    int32_t lds_buffers; /* 2 */
+} ffgemm_tile;
+
+/* Library / device bring-up.  Optional: every entry point initialises lazily. */
+int ffgemm_init(void);
+void ffgemm_shutdown(void);
+const char *ffgemm_status_string(int status);
+const char *ffgemm_last_error(void);
+const char *ffgemm_version(void);
+
+/* Checks a problem and the tile the library would use for it; no launch.
+ * Replaces GemmMfmaKernelWriter::checkKernelParameters
+ * (src/gemmMfmaKernelWriter.h:321-405). */
+int ffgemm_query_tile(const ffgemm_problem *p, ffgemm_tile *out);
+
+/* D = alpha*op(A)*op(B) + beta*C on `stream` (hipStream_t, NULL = null stream).
+ * Replaces the path GemmMfmaAsmSolution::generateKernel -> StreamHip::Launch
+ * (src/gemmMfma.cpp:24-185, ff_gpu_src/ff_hip_runtime.cpp:278-320).
+ * `tile` may be NULL.  beta == 0 never reads C. */
+int ffgemm_run(const ffgemm_problem *p, const void *A, const void *B, const void *C, void *D,
+               const ffgemm_tile *tile, void *stream);
+
+/* The reference kernel's argument list, in its kernarg order
+ * D, C, A, B, alpha, beta, ldd, bsD, ldc, bsC, lda, bsA, ldb, bsB
+ * (src/gemmMfma.cpp:106-119, src/gemmMfmaKernelWriter.h:323-338), TN layout,
+ * plus the sizes and dtype that the reference baked into the generated kernel.
+ * dtype is the reference's -d code: 1 fp32, 2 fp16, 3 bf16. */
+int ffgemm_run_tn_kernarg(void *D, const void *C, const void *A, const void *B, float alpha,
+                          float beta, uint32_t ldd, uint32_t bsD, uint32_t ldc, uint32_t bsC,
+                          uint32_t lda, uint32_t bsA, uint32_t ldb, uint32_t bsB, uint32_t M,
+                          uint32_t N, uint32_t K, uint32_t batch, int dtype,
+                          const ffgemm_tile *tile, void *stream);
+
+/* FP8 e4m3 per-tensor quantisation: scale = 448 / max|x| (fp32, computed on
+ * the device into *scale_out), q = rne_sat(x * scale).  x is an M x K (or
+ * K x N) fp32 matrix given in `layout_of_x` terms: for operand 'A' with
+ * trans_x = 0 the input is M-contiguous (lda x K), trans_x = 1 K-contiguous.
+ * q is written K-contiguous ("T" for A, "N" for B) with leading dimension
+ * ldq, as the FP8 GEMM (TN only) consumes it.  `work` is a device scratch of
+ * at least ffgemm_fp8_workspace_size() bytes. */
+size_t ffgemm_fp8_workspace_size(void);
+int ffgemm_fp8_quantize(const float *x, int64_t rows, int64_t k, int64_t ldx, int trans_x,
+                        uint8_t *q, int64_t ldq, float *scale_out, void *work, void *stream);
+
+/* Elementwise fp32 -> bf16/fp16 conversion on the device (RNE), n elements. */
+int ffgemm_convert(const float *src, void *dst, int64_t n, int dtype_dst, void *stream);
+
+/* Grouped GEMM: `count` independent problems (same dtype, layout, compute)
+ * run by one persistent launch.  A plan uploads the problem descriptors once
+ * (this call allocates device memory and synchronises), after which
+ * ffgemm_grouped_plan_run only launches a kernel and can be captured into a
+ * hipGraph.  Pointer arrays are host arrays of device pointers. */
+typedef struct ffgemm_grouped_plan_s *ffgemm_grouped_plan;
+int ffgemm_grouped_plan_create(ffgemm_grouped_plan *plan, int count, const ffgemm_problem *probs,
+                               const void *const *A, const void *const *B,
+                               const void *const *C, void *const *D, const ffgemm_tile *tile);
+int ffgemm_grouped_plan_run(ffgemm_grouped_plan plan, void *stream);
+int ffgemm_grouped_plan_info(ffgemm_grouped_plan plan, int64_t *total_tiles, int32_t *grid,
+                             ffgemm_tile *tile);
+int ffgemm_grouped_plan_destroy(ffgemm_grouped_plan plan);
+
+/* Number of compute units of the current device (for tile scheduling). */
+int ffgemm_device_cu_count(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,135 @@
+// Auxiliary device kernels: FP8 per-tensor quantisation, fp32 -> bf16/fp16
+// conversion, and the FP64 MFMA GEMM used as the error oracle.
+#include "aux_kernels.hpp"
+
+namespace ffgemm {
+
+// Workspace layout for FP8 quantisation: [0] amax bits (uint32).
+__global__ void zero_u32_kernel(uint32_t* p, int n) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = 0u;
+}
+
+// max |x| over a rows x k matrix with leading dimension ldx; element (r, kk)
+// at kk + r*ldx when trans_x (K-contiguous) else r + kk*ldx.  The order of
+// traversal does not matter for a maximum.
+__global__ void amax_kernel(const float* __restrict__ x, int64_t rows, int64_t k, int64_t ldx,
+                            int trans_x, uint32_t* __restrict__ amax_bits) {
+    const int64_t inner = trans_x ? k : rows;
+    const int64_t outer = trans_x ? rows : k;
+    const int64_t total = inner * outer;
+    float m = 0.0f;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < total;
+         i += (int64_t)gridDim.x * blockDim.x) {
+        int64_t o = i / inner, in = i - o * inner;
+        float v = fabsf(x[o * ldx + in]);
+        m = v > m ? v : m;  // NaN inputs are ignored
+    }
+    // wave reduction then one atomic per wave (non-negative floats order like their bits)
+    for (int off = 32; off > 0; off >>= 1) {
+        float o = __shfl_xor(m, off, 64);
+        m = o > m ? o : m;
+    }
+    if ((threadIdx.x & 63) == 0) atomicMax(amax_bits, __builtin_bit_cast(uint32_t, m));
+}
+
+__global__ void fp8_scale_kernel(const uint32_t* __restrict__ amax_bits, float* __restrict__ scale) {
+    if (threadIdx.x == 0 && blockIdx.x == 0) {
+        float a = __builtin_bit_cast(float, *amax_bits);
+        scale[0] = (a > 0.0f) ? 448.0f / a : 1.0f;
+    }
+}
+
+__device__ inline uint8_t f32_to_e4m3(float v) {
+    // saturate to the e4m3 range, then the hardware RNE conversion
+    v = fminf(fmaxf(v, -448.0f), 448.0f);
+    int p = __builtin_amdgcn_cvt_pk_fp8_f32(v, 0.0f, 0, false);
+    return (uint8_t)(p & 0xff);
+}
+
+// q[kk + r*ldq] = e4m3(x(r, kk) * scale).  32 x 32 tiles through LDS so both
+// the read and the write are coalesced when the layouts differ.
+__global__ void fp8_quant_kernel(const float* __restrict__ x, int64_t rows, int64_t k, int64_t ldx,
+                                 int trans_x, uint8_t* __restrict__ q, int64_t ldq,
+                                 const float* __restrict__ scale) {
+    __shared__ float tile[32][33];
+    const float s = *scale;
+    const int64_t r0 = (int64_t)blockIdx.y * 32, k0 = (int64_t)blockIdx.x * 32;
+    const int tx = threadIdx.x & 31, ty = threadIdx.x >> 5;  // 32 x 8
+    for (int yy = ty; yy < 32; yy += 8) {
+        int64_t r, kk;
+        if (trans_x) { r = r0 + yy; kk = k0 + tx; } else { r = r0 + tx; kk = k0 + yy; }
+        float v = 0.0f;
+        if (r < rows && kk < k) v = trans_x ? x[r * ldx + kk] : x[kk * ldx + r];
+        if (trans_x) tile[yy][tx] = v; else tile[tx][yy] = v;
+    }
+    __syncthreads();
+    for (int yy = ty; yy < 32; yy += 8) {
+        int64_t r = r0 + yy, kk = k0 + tx;
+        if (r < rows && kk < k) q[r * ldq + kk] = f32_to_e4m3(tile[yy][tx] * s);
+    }
+}
+
+__global__ void convert_kernel(const float* __restrict__ src, void* __restrict__ dst, int64_t n, int to) {
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+        float v = src[i];
+        if (to == 3) ((__bf16*)dst)[i] = (__bf16)v;
+        else ((_Float16*)dst)[i] = (_Float16)v;
+    }
+}
+
+// FP64 oracle GEMM on v_mfma_f64_16x16x4_f64: 64 x 64 tile per 256-thread
+// block, each wave a 32 x 32 quarter (2 x 2 MFMA tiles), K staged through LDS
+// 16 at a time.  Same column-major problem and layouts as the fp32 path.
+__global__ void __launch_bounds__(256) gemm_f64_kernel(F64Args a) {
+    __shared__ double As[64][17];
+    __shared__ double Bs[64][17];
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int wm = wave & 1, wn = wave >> 1;
+    const int64_t m0 = (int64_t)blockIdx.x * 64, n0 = (int64_t)blockIdx.y * 64;
+    const int b = blockIdx.z;
+    const double* A = a.A + (int64_t)b * a.bsA;
+    const double* B = a.B + (int64_t)b * a.bsB;
+    f64x4_t acc[2][2];
+    for (int i = 0; i < 2; ++i)
+        for (int j = 0; j < 2; ++j)
+            for (int e = 0; e < 4; ++e) acc[i][j][e] = 0.0;
+    for (int64_t k0 = 0; k0 < a.K; k0 += 16) {
+        for (int idx = threadIdx.x; idx < 64 * 16; idx += 256) {
+            int r = idx & 63, kk = idx >> 6;
+            int64_t m = m0 + r, n = n0 + r, k = k0 + kk;
+            double av = 0.0, bv = 0.0;
+            if (m < a.M && k < a.K) av = a.transA ? A[k + m * a.lda] : A[m + k * a.lda];
+            if (n < a.N && k < a.K) bv = a.transB ? B[n + k * a.ldb] : B[k + n * a.ldb];
+            As[r][kk] = av;
+            Bs[r][kk] = bv;
+        }
+        __syncthreads();
+        for (int ks = 0; ks < 16; ks += 4) {
+            for (int i = 0; i < 2; ++i)
+                for (int j = 0; j < 2; ++j) {
+                    // lane l supplies A[row l%16][k l/16] and B[k l/16][col l%16]
+                    double av = As[wm * 32 + i * 16 + (lane & 15)][ks + (lane >> 4)];
+                    double bv = Bs[wn * 32 + j * 16 + (lane & 15)][ks + (lane >> 4)];
+                    acc[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(av, bv, acc[i][j], 0, 0, 0);
+                }
+        }
+        __syncthreads();
+    }
+    double* D = a.D + (int64_t)b * a.bsD;
+    const double* C = a.C ? a.C + (int64_t)b * a.bsC : nullptr;
+    for (int i = 0; i < 2; ++i)
+        for (int j = 0; j < 2; ++j)
+            for (int e = 0; e < 4; ++e) {
+                // output row 4*(l/16) + e, column l%16 of the 16 x 16 tile
+                int64_t m = m0 + wm * 32 + i * 16 + 4 * (lane >> 4) + e;
+                int64_t n = n0 + wn * 32 + j * 16 + (lane & 15);
+                if (m < a.M && n < a.N) {
+                    double r = a.alpha * acc[i][j][e];
+                    if (a.beta != 0.0) r += a.beta * C[m + n * a.ldc];
+                    D[m + n * a.ldd] = r;
+                }
+            }
+}
+
+}  // namespace ffgemm

@@ -1,0 +1,27 @@
+// Declarations of the auxiliary kernels (aux_kernels.hip).
+#pragma once
+#include "gemm_common.hpp"
+
+namespace ffgemm {
+
+struct F64Args {
+    const double* A;
+    const double* B;
+    const double* C;
+    double* D;
+    int64_t M, N, K, lda, ldb, ldc, ldd, bsA, bsB, bsC, bsD;
+    double alpha, beta;
+    int transA, transB;
+};
+
+__global__ void zero_u32_kernel(uint32_t* p, int n);
+__global__ void amax_kernel(const float* __restrict__ x, int64_t rows, int64_t k, int64_t ldx,
+                            int trans_x, uint32_t* __restrict__ amax_bits);
+__global__ void fp8_scale_kernel(const uint32_t* __restrict__ amax_bits, float* __restrict__ scale);
+__global__ void fp8_quant_kernel(const float* __restrict__ x, int64_t rows, int64_t k, int64_t ldx,
+                                 int trans_x, uint8_t* __restrict__ q, int64_t ldq,
+                                 const float* __restrict__ scale);
+__global__ void convert_kernel(const float* __restrict__ src, void* __restrict__ dst, int64_t n, int to);
+__global__ void __launch_bounds__(256) gemm_f64_kernel(F64Args a);
+
+}  // namespace ffgemm

@@ -1,0 +1,478 @@
+// C ABI of libffgemm.so (include/ffgemm.h): problem validation, tile choice
+// and kernel launch.  Replaces GemmMfmaAsmSolution::generateKernel
+// (src/gemmMfma.cpp:24-185), GemmMfmaKernelWriter::checkKernelParameters
+// (src/gemmMfmaKernelWriter.h:321-405) and StreamHip::Launch
+// (ff_gpu_src/ff_hip_runtime.cpp:278-320) for the gemmMfma path.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "../../include/ffgemm.h"
+#include "aux_kernels.hpp"
+#include "grouped.hpp"
+
+using namespace ffgemm;
+
+namespace {
+
+thread_local std::string g_last_error;
+
+int fail(int status, const char* fmt, ...) {
+    char buf[512];
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(buf, sizeof(buf), fmt, ap);
+    va_end(ap);
+    g_last_error = buf;
+    return status;
+}
+
+int hip_fail(hipError_t e, const char* what) {
+    return fail(FFGEMM_FATAL, "%s: %s", what, hipGetErrorString(e));
+}
+
+std::mutex g_dev_mu;
+int g_cu_count[64] = {0};
+
+int cu_count() {
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= 64) return 256;
+    std::lock_guard<std::mutex> lk(g_dev_mu);
+    if (g_cu_count[dev] == 0) {
+        int n = 0;
+        if (hipDeviceGetAttribute(&n, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || n <= 0) n = 256;
+        g_cu_count[dev] = n;
+    }
+    return g_cu_count[dev];
+}
+
+struct Tile {
+    int m, n;
+};
+const Tile kTiles[] = {{256, 256}, {256, 128}, {128, 256}, {128, 128}, {64, 64}};
+
+bool tile_supported(int m, int n) {
+    for (const Tile& t : kTiles)
+        if (t.m == m && t.n == n) return true;
+    return false;
+}
+
+// Resident workgroups per CU for a tile (VGPR/AGPR and LDS limited).
+int tile_occupancy(int m, int n) {
+    if (m == 64 && n == 64) return 4;
+    if (m == 128 && n == 128) return 2;
+    return 1;
+}
+
+int64_t cdiv(int64_t a, int64_t b) { return (a + b - 1) / b; }
+
+// Largest tile whose grid still fills every CU; else the one with most tiles.
+Tile pick_tile(int64_t M, int64_t N, int64_t batch) {
+    const int64_t cus = cu_count();
+    for (const Tile& t : kTiles) {
+        if (cdiv(M, t.m) * cdiv(N, t.n) * batch >= cus) return t;
+    }
+    return kTiles[4];
+}
+
+int default_group_m(int64_t tiles_m, int64_t tiles_n) {
+    int64_t per_xcd = std::max<int64_t>(1, tiles_m * tiles_n / 8);
+    int g = 1;
+    while ((int64_t)(g * 2) * (g * 2) <= per_xcd && g * 2 <= tiles_m) g *= 2;
+    return g;
+}
+
+int out_kind_of(int dtype, int dtype_out) {
+    int o = dtype_out ? dtype_out : (dtype == FFGEMM_F8E4M3 ? FFGEMM_F32 : dtype);
+    if (o == FFGEMM_F32) return kOutF32;
+    if (o == FFGEMM_F16) return kOutF16;
+    if (o == FFGEMM_BF16) return kOutBF16;
+    return -1;
+}
+
+int elem_bytes(int dtype) {
+    switch (dtype) {
+        case FFGEMM_F32: return 4;
+        case FFGEMM_F16: case FFGEMM_BF16: return 2;
+        case FFGEMM_F8E4M3: return 1;
+        case FFGEMM_F64: return 8;
+    }
+    return 0;
+}
+
+struct Launch {
+    int mode;
+    int out;
+    int layout;
+    Tile tile;
+    TileArgs args;
+    int64_t grid;
+};
+
+bool aligned16(const void* p) { return ((uintptr_t)p & 15u) == 0; }
+
+// Checks the problem against what the kernels support and fills `L`.
+int prepare(const ffgemm_problem* p, const void* A, const void* B, const void* C, void* D,
+            const ffgemm_tile* tile, Launch& L) {
+    if (!p) return fail(FFGEMM_ERR, "null problem");
+    if (p->layout < 0 || p->layout > 3) return fail(FFGEMM_ERR, "bad layout %d", p->layout);
+    const int64_t M = p->M, N = p->N, K = p->K;
+    const int64_t batch = p->batch <= 0 ? 1 : p->batch;
+    if (M < 0 || N < 0 || K < 0) return fail(FFGEMM_ERR, "negative size");
+    const int64_t lim = (int64_t)1 << 31;
+    if (M >= lim || N >= lim || K >= lim || batch >= lim) return fail(FFGEMM_ERR, "size exceeds 2^31");
+    const bool ta = p->layout >= 2, tb = (p->layout & 1) != 0;
+    int mode;
+    switch (p->dtype) {
+        case FFGEMM_BF16: mode = kBF16; break;
+        case FFGEMM_F16: mode = kF16; break;
+        case FFGEMM_F32: mode = (p->compute == FFGEMM_COMPUTE_BF16X3) ? kX3 : kF32; break;
+        case FFGEMM_F8E4M3: mode = kF8; break;
+        default: return fail(FFGEMM_ERR, "unsupported dtype %d", p->dtype);
+    }
+    if (p->compute == FFGEMM_COMPUTE_BF16X3 && p->dtype != FFGEMM_F32)
+        return fail(FFGEMM_ERR, "bf16x3 compute needs fp32 operands");
+    const int out = out_kind_of(p->dtype, p->dtype_out);
+    if (out < 0) return fail(FFGEMM_ERR, "unsupported output dtype %d", p->dtype_out);
+    if ((mode == kBF16 && out == kOutF16) || (mode == kF16 && out == kOutBF16) ||
+        ((mode == kF32 || mode == kX3) && out != kOutF32) || (mode == kF8 && out == kOutF16))
+        return fail(FFGEMM_ERR, "unsupported input/output dtype pair %d/%d", p->dtype, p->dtype_out);
+    if (mode == kF8 && p->layout != FFGEMM_TN) return fail(FFGEMM_ERR, "fp8 supports the TN layout only");
+
+    // leading dimensions (column-major)
+    const int64_t rowsA = ta ? K : M, rowsB = tb ? N : K;
+    if (p->lda < std::max<int64_t>(1, rowsA)) return fail(FFGEMM_ERR, "lda %lld < %lld", (long long)p->lda, (long long)rowsA);
+    if (p->ldb < std::max<int64_t>(1, rowsB)) return fail(FFGEMM_ERR, "ldb %lld < %lld", (long long)p->ldb, (long long)rowsB);
+    if (p->ldd < std::max<int64_t>(1, M)) return fail(FFGEMM_ERR, "ldd %lld < M", (long long)p->ldd);
+    const bool use_c = p->beta != 0.0f;
+    if (use_c && p->ldc < std::max<int64_t>(1, M)) return fail(FFGEMM_ERR, "ldc %lld < M", (long long)p->ldc);
+    if (p->lda >= lim || p->ldb >= lim || p->ldc >= lim || p->ldd >= lim) return fail(FFGEMM_ERR, "ld exceeds 2^31");
+
+    if (M == 0 || N == 0) {  // nothing to compute or store
+        L.mode = mode; L.out = out; L.layout = p->layout; L.tile = kTiles[4]; L.grid = 0;
+        memset(&L.args, 0, sizeof(L.args));
+        return FFGEMM_SUCCESS;
+    }
+    const int es = mode_elem_bytes(mode);
+    const int ve = 16 / es;          // elements per 16-byte vector
+    const int os = out == kOutF32 ? 4 : 2;
+    if ((p->lda * es) % 16 || (p->ldb * es) % 16)
+        return fail(FFGEMM_ERR, "lda/ldb must be multiples of %d elements (16-byte rows)", ve);
+    if ((p->ldd * os) % 16 || (use_c && (p->ldc * os) % 16))
+        return fail(FFGEMM_ERR, "ldc/ldd must be multiples of %d elements", 16 / os);
+    if ((ta || !tb) && K % ve)
+        return fail(FFGEMM_ERR, "K must be a multiple of %d for a K-contiguous operand", ve);
+    if (M && N && (!A || !B || !D)) return fail(FFGEMM_ERR, "null matrix pointer");
+    if (use_c && M && N && !C) return fail(FFGEMM_ERR, "beta != 0 but C is null");
+    if (!aligned16(A) || !aligned16(B) || !aligned16(D) || (use_c && !aligned16(C)))
+        return fail(FFGEMM_ERR, "matrix pointers must be 16-byte aligned");
+    if (batch > 1) {
+        if ((p->bsA * es) % 16 || (p->bsB * es) % 16 || (p->bsD * os) % 16 || (use_c && (p->bsC * os) % 16))
+            return fail(FFGEMM_ERR, "batch strides must keep 16-byte alignment");
+    }
+    // every operand must be addressable with 31-bit byte offsets
+    const int64_t exA = ((ta ? M : K) + 256) * p->lda * es;
+    const int64_t exB = ((tb ? K : N) + 256) * p->ldb * es;
+    if (exA >= lim || exB >= lim) return fail(FFGEMM_ERR, "operand larger than 2 GiB per batch");
+
+    Tile t = pick_tile(M, N, batch);
+    int group_m = 0;
+    if (tile && tile->mt_m) {
+        if (!tile_supported(tile->mt_m, tile->mt_n))
+            return fail(FFGEMM_ERR, "unsupported macro tile %dx%d", tile->mt_m, tile->mt_n);
+        if ((tile->waves_m && tile->waves_m != 2) || (tile->waves_n && tile->waves_n != 2) ||
+            (tile->mfma_mn && tile->mfma_mn != 32) || (tile->lds_buffers && tile->lds_buffers != 2) ||
+            (tile->depth_u && tile->depth_u != mode_depth_u(mode)))
+            return fail(FFGEMM_ERR, "unsupported tile knobs (waves 2x2, mfma 32, 2 LDS buffers, depth_u %d)",
+                        mode_depth_u(mode));
+        t = Tile{tile->mt_m, tile->mt_n};
+        group_m = tile->group_m;
+    }
+    const int64_t tiles_m = cdiv(M, t.m), tiles_n = cdiv(N, t.n);
+    const int64_t grid = tiles_m * tiles_n * batch;
+    if (grid >= lim) return fail(FFGEMM_ERR, "too many tiles");
+
+    TileArgs& a = L.args;
+    memset(&a, 0, sizeof(a));
+    a.A = A; a.B = B; a.C = use_c ? C : nullptr; a.D = D;
+    a.bsA = p->bsA; a.bsB = p->bsB; a.bsC = p->bsC; a.bsD = p->bsD;
+    a.M = (int32_t)M; a.N = (int32_t)N; a.K = (int32_t)K;
+    a.lda = (int32_t)p->lda; a.ldb = (int32_t)p->ldb; a.ldc = (int32_t)(use_c ? p->ldc : p->ldd); a.ldd = (int32_t)p->ldd;
+    a.alpha = p->alpha; a.beta = p->beta;
+    a.scaleA = p->scaleA; a.scaleB = p->scaleB;
+    a.tiles_m = (int32_t)tiles_m; a.tiles_n = (int32_t)tiles_n; a.batch = (int32_t)batch;
+    a.group_m = group_m > 0 ? group_m : default_group_m(tiles_m, tiles_n);
+    a.flags = (use_c ? kFlagBeta : 0u) | (p->round == FFGEMM_ROUND_TRUNCATE ? kFlagTrunc : 0u);
+    L.mode = mode; L.out = out; L.layout = p->layout; L.tile = t; L.grid = grid;
+    return FFGEMM_SUCCESS;
+}
+
+int launch(const Launch& L, hipStream_t s) {
+    int r = -1;
+    switch (L.mode) {
+        case kBF16: r = launch_bf16(L.layout, L.out, L.tile.m, L.tile.n, L.args, (int)L.grid, s); break;
+        case kF16: r = launch_f16(L.layout, L.out, L.tile.m, L.tile.n, L.args, (int)L.grid, s); break;
+        case kF32: r = launch_f32(L.layout, L.out, L.tile.m, L.tile.n, L.args, (int)L.grid, s); break;
+        case kX3: r = launch_x3(L.layout, L.out, L.tile.m, L.tile.n, L.args, (int)L.grid, s); break;
+        case kF8: r = launch_f8(L.layout, L.out, L.tile.m, L.tile.n, L.args, (int)L.grid, s); break;
+    }
+    if (r != 0) return fail(FFGEMM_ERR, "no kernel for mode %d out %d layout %d tile %dx%d", L.mode, L.out,
+                            L.layout, L.tile.m, L.tile.n);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return hip_fail(e, "kernel launch");
+    return FFGEMM_SUCCESS;
+}
+
+int run_f64(const ffgemm_problem* p, const void* A, const void* B, const void* C, void* D, hipStream_t s) {
+    const int64_t batch = p->batch <= 0 ? 1 : p->batch;
+    if (p->M < 0 || p->N < 0 || p->K < 0) return fail(FFGEMM_ERR, "negative size");
+    if (p->layout < 0 || p->layout > 3) return fail(FFGEMM_ERR, "bad layout %d", p->layout);
+    const bool ta = p->layout >= 2, tb = (p->layout & 1) != 0;
+    if (p->lda < (ta ? p->K : p->M) || p->ldb < (tb ? p->N : p->K) || p->ldd < p->M ||
+        (p->beta != 0.0f && p->ldc < p->M))
+        return fail(FFGEMM_ERR, "leading dimension too small");
+    if (p->M == 0 || p->N == 0) return FFGEMM_SUCCESS;
+    if (batch > 65535) return fail(FFGEMM_ERR, "batch too large for the fp64 kernel");
+    F64Args a;
+    a.A = (const double*)A; a.B = (const double*)B; a.C = p->beta != 0.0f ? (const double*)C : nullptr;
+    a.D = (double*)D;
+    a.M = p->M; a.N = p->N; a.K = p->K; a.lda = p->lda; a.ldb = p->ldb; a.ldc = p->ldc; a.ldd = p->ldd;
+    a.bsA = p->bsA; a.bsB = p->bsB; a.bsC = p->bsC; a.bsD = p->bsD;
+    a.alpha = p->alpha; a.beta = p->beta; a.transA = ta; a.transB = tb;
+    dim3 grid((unsigned)cdiv(p->M, 64), (unsigned)cdiv(p->N, 64), (unsigned)batch);
+    hipLaunchKernelGGL(gemm_f64_kernel, grid, dim3(256), 0, s, a);
+    hipError_t e = hipGetLastError();
+    return e == hipSuccess ? FFGEMM_SUCCESS : hip_fail(e, "fp64 kernel launch");
+}
+
+}  // namespace
+
+extern "C" {
+
+int ffgemm_init(void) {
+    int dev = 0;
+    hipError_t e = hipGetDevice(&dev);
+    if (e != hipSuccess) return hip_fail(e, "hipGetDevice");
+    cu_count();
+    return FFGEMM_SUCCESS;
+}
+
+void ffgemm_shutdown(void) {}
+
+const char* ffgemm_status_string(int status) {
+    switch (status) {
+        case FFGEMM_SUCCESS: return "SUCCESS";
+        case FFGEMM_WARN: return "RTN_WARN";
+        case FFGEMM_ERR: return "RTN_ERR";
+        case FFGEMM_FATAL: return "RTN_FATAL";
+    }
+    return "UNKNOWN";
+}
+
+const char* ffgemm_last_error(void) { return g_last_error.c_str(); }
+
+const char* ffgemm_version(void) { return "ffgemm 0.1 gfx950"; }
+
+int ffgemm_device_cu_count(void) { return cu_count(); }
+
+int ffgemm_query_tile(const ffgemm_problem* p, ffgemm_tile* out) {
+    if (p && p->dtype == FFGEMM_F64) {
+        if (out) { memset(out, 0, sizeof(*out)); out->mt_m = 64; out->mt_n = 64; out->depth_u = 16; }
+        return FFGEMM_SUCCESS;
+    }
+    // validation with dummy aligned pointers; nothing is dereferenced
+    static const int64_t dummy[4] = {0, 0, 0, 0};
+    Launch L;
+    int r = prepare(p, dummy, dummy, dummy, (void*)dummy, out && out->mt_m ? out : nullptr, L);
+    if (r != FFGEMM_SUCCESS) return r;
+    if (out) {
+        memset(out, 0, sizeof(*out));
+        out->mt_m = L.tile.m; out->mt_n = L.tile.n; out->depth_u = mode_depth_u(L.mode);
+        out->waves_m = 2; out->waves_n = 2; out->mfma_mn = 32; out->group_m = L.args.group_m; out->lds_buffers = 2;
+    }
+    return FFGEMM_SUCCESS;
+}
+
+int ffgemm_run(const ffgemm_problem* p, const void* A, const void* B, const void* C, void* D,
+               const ffgemm_tile* tile, void* stream) {
+    if (p && p->dtype == FFGEMM_F64) return run_f64(p, A, B, C, D, (hipStream_t)stream);
+    Launch L;
+    int r = prepare(p, A, B, C, D, tile, L);
+    if (r != FFGEMM_SUCCESS) return r;
+    if (p->M == 0 || p->N == 0) return FFGEMM_SUCCESS;
+    return launch(L, (hipStream_t)stream);
+}
+
+int ffgemm_run_tn_kernarg(void* D, const void* C, const void* A, const void* B, float alpha, float beta,
+                          uint32_t ldd, uint32_t bsD, uint32_t ldc, uint32_t bsC, uint32_t lda, uint32_t bsA,
+                          uint32_t ldb, uint32_t bsB, uint32_t M, uint32_t N, uint32_t K, uint32_t batch,
+                          int dtype, const ffgemm_tile* tile, void* stream) {
+    if (dtype < FFGEMM_F32 || dtype > FFGEMM_BF16) return fail(FFGEMM_ERR, "data type %d not in {1,2,3}", dtype);
+    ffgemm_problem p;
+    memset(&p, 0, sizeof(p));
+    p.layout = FFGEMM_TN; p.dtype = dtype; p.dtype_out = dtype;
+    p.M = M; p.N = N; p.K = K; p.batch = batch ? batch : 1;
+    p.lda = lda; p.ldb = ldb; p.ldc = ldc; p.ldd = ldd;
+    p.bsA = bsA; p.bsB = bsB; p.bsC = bsC; p.bsD = bsD;
+    p.alpha = alpha; p.beta = beta;
+    return ffgemm_run(&p, A, B, C, D, tile, stream);
+}
+
+size_t ffgemm_fp8_workspace_size(void) { return 256; }
+
+int ffgemm_fp8_quantize(const float* x, int64_t rows, int64_t k, int64_t ldx, int trans_x, uint8_t* q,
+                        int64_t ldq, float* scale_out, void* work, void* stream) {
+    if (rows < 0 || k < 0) return fail(FFGEMM_ERR, "negative size");
+    if (!x || !q || !scale_out || !work) return fail(FFGEMM_ERR, "null pointer");
+    if (ldx < (trans_x ? k : rows) || ldq < k) return fail(FFGEMM_ERR, "leading dimension too small");
+    if (rows >= (1ll << 31) / 32 || k >= (1ll << 31) / 32) return fail(FFGEMM_ERR, "matrix too large");
+    hipStream_t s = (hipStream_t)stream;
+    uint32_t* amax = (uint32_t*)work;
+    hipLaunchKernelGGL(zero_u32_kernel, dim3(1), dim3(64), 0, s, amax, 1);
+    const int64_t total = rows * k;
+    int blocks = (int)std::min<int64_t>(std::max<int64_t>(1, cdiv(total, 256 * 8)), 2048);
+    if (total > 0) hipLaunchKernelGGL(amax_kernel, dim3(blocks), dim3(256), 0, s, x, rows, k, ldx, trans_x, amax);
+    hipLaunchKernelGGL(fp8_scale_kernel, dim3(1), dim3(64), 0, s, (const uint32_t*)amax, scale_out);
+    if (total > 0) {
+        dim3 grid((unsigned)cdiv(k, 32), (unsigned)cdiv(rows, 32));
+        hipLaunchKernelGGL(fp8_quant_kernel, grid, dim3(256), 0, s, x, rows, k, ldx, trans_x, q, ldq,
+                           (const float*)scale_out);
+    }
+    hipError_t e = hipGetLastError();
+    return e == hipSuccess ? FFGEMM_SUCCESS : hip_fail(e, "fp8 quantize launch");
+}
+
+int ffgemm_convert(const float* src, void* dst, int64_t n, int dtype_dst, void* stream) {
+    if (dtype_dst != FFGEMM_BF16 && dtype_dst != FFGEMM_F16) return fail(FFGEMM_ERR, "convert to bf16/f16 only");
+    if (n < 0 || (n && (!src || !dst))) return fail(FFGEMM_ERR, "bad arguments");
+    if (n == 0) return FFGEMM_SUCCESS;
+    int blocks = (int)std::min<int64_t>(cdiv(n, 256 * 4), 8192);
+    hipLaunchKernelGGL(convert_kernel, dim3(blocks), dim3(256), 0, (hipStream_t)stream, src, dst, n, dtype_dst);
+    hipError_t e = hipGetLastError();
+    return e == hipSuccess ? FFGEMM_SUCCESS : hip_fail(e, "convert launch");
+}
+
+}  // extern "C"
+
+// ---------------------------------------------------------------------------
+// Grouped GEMM plans
+// ---------------------------------------------------------------------------
+struct ffgemm_grouped_plan_s {
+    GroupDesc* d_descs = nullptr;
+    int count = 0;
+    int64_t total = 0;
+    int grid = 0;
+    int mode = 0, out = 0, layout = 0;
+    Tile tile{0, 0};
+    int device = 0;
+};
+
+extern "C" {
+
+int ffgemm_grouped_plan_create(ffgemm_grouped_plan* plan, int count, const ffgemm_problem* probs,
+                               const void* const* A, const void* const* B, const void* const* C,
+                               void* const* D, const ffgemm_tile* tile) {
+    if (!plan) return fail(FFGEMM_ERR, "null plan pointer");
+    *plan = nullptr;
+    if (count <= 0 || !probs || !A || !B || !D) return fail(FFGEMM_ERR, "empty or null group");
+    // one tile for the whole group: the caller's, else the largest that fills the CUs
+    Tile t{0, 0};
+    if (tile && tile->mt_m) {
+        if (!tile_supported(tile->mt_m, tile->mt_n))
+            return fail(FFGEMM_ERR, "unsupported macro tile %dx%d", tile->mt_m, tile->mt_n);
+        t = Tile{tile->mt_m, tile->mt_n};
+        if (t.m == 256 && t.n == 256) return fail(FFGEMM_ERR, "grouped GEMM supports tiles up to 256x128");
+    } else {
+        for (const Tile& c : kTiles) {
+            if (c.m == 256 && c.n == 256) continue;  // no grouped kernel for this tile
+            int64_t n = 0;
+            for (int i = 0; i < count; ++i) n += cdiv(probs[i].M, c.m) * cdiv(probs[i].N, c.n);
+            t = c;
+            if (n >= 2 * cu_count()) break;
+        }
+    }
+    ffgemm_tile tt;
+    memset(&tt, 0, sizeof(tt));
+    tt.mt_m = t.m; tt.mt_n = t.n;
+    std::vector<GroupDesc> descs(count);
+    std::vector<int> order(count);
+    int mode = -1, out = -1, layout = -1;
+    for (int i = 0; i < count; ++i) {
+        const ffgemm_problem& p = probs[i];
+        if ((p.batch > 1)) return fail(FFGEMM_ERR, "grouped problems must have batch 1");
+        Launch L;
+        int r = prepare(&p, A[i], B[i], C ? C[i] : nullptr, D[i], &tt, L);
+        if (r != FFGEMM_SUCCESS) return r;
+        if (i == 0) { mode = L.mode; out = L.out; layout = L.layout; }
+        else if (L.mode != mode || L.out != out || L.layout != layout)
+            return fail(FFGEMM_ERR, "grouped problems must share dtype, compute and layout");
+        descs[i].a = L.args;
+        descs[i].a.group_m = 1;
+        order[i] = i;
+    }
+    // longest tiles (largest K) first, so short tiles fill the tail of the schedule
+    std::stable_sort(order.begin(), order.end(), [&](int x, int y) { return probs[x].K > probs[y].K; });
+    std::vector<GroupDesc> sorted;
+    int64_t total = 0;
+    for (int i : order) {
+        GroupDesc d = descs[i];
+        int64_t nt = (int64_t)d.a.tiles_m * d.a.tiles_n;
+        if (nt == 0) continue;
+        d.tile_start = total;
+        total += nt;
+        sorted.push_back(d);
+    }
+    ffgemm_grouped_plan_s* P = new ffgemm_grouped_plan_s();
+    P->count = (int)sorted.size();
+    P->total = total;
+    P->grid = (int)std::max<int64_t>(1, std::min<int64_t>(total, (int64_t)cu_count() * tile_occupancy(t.m, t.n)));
+    P->mode = mode; P->out = out; P->layout = layout; P->tile = t;
+    hipGetDevice(&P->device);
+    if (P->count > 0) {
+        hipError_t e = hipMalloc((void**)&P->d_descs, sizeof(GroupDesc) * P->count);
+        if (e != hipSuccess) { delete P; return hip_fail(e, "hipMalloc(descriptors)"); }
+        e = hipMemcpy(P->d_descs, sorted.data(), sizeof(GroupDesc) * P->count, hipMemcpyHostToDevice);
+        if (e != hipSuccess) { hipFree(P->d_descs); delete P; return hip_fail(e, "hipMemcpy(descriptors)"); }
+    }
+    *plan = P;
+    return FFGEMM_SUCCESS;
+}
+
+int ffgemm_grouped_plan_run(ffgemm_grouped_plan plan, void* stream) {
+    if (!plan) return fail(FFGEMM_ERR, "null plan");
+    if (plan->total == 0) return FFGEMM_SUCCESS;
+    int r = launch_grouped(plan->mode, plan->out, plan->layout, plan->tile.m, plan->tile.n, plan->d_descs,
+                           plan->count, plan->total, plan->grid, (hipStream_t)stream);
+    if (r != 0) return fail(FFGEMM_ERR, "no grouped kernel for mode %d out %d layout %d", plan->mode, plan->out,
+                            plan->layout);
+    hipError_t e = hipGetLastError();
+    return e == hipSuccess ? FFGEMM_SUCCESS : hip_fail(e, "grouped launch");
+}
+
+int ffgemm_grouped_plan_info(ffgemm_grouped_plan plan, int64_t* total_tiles, int32_t* grid, ffgemm_tile* tile) {
+    if (!plan) return fail(FFGEMM_ERR, "null plan");
+    if (total_tiles) *total_tiles = plan->total;
+    if (grid) *grid = plan->grid;
+    if (tile) {
+        memset(tile, 0, sizeof(*tile));
+        tile->mt_m = plan->tile.m; tile->mt_n = plan->tile.n; tile->waves_m = 2; tile->waves_n = 2;
+        tile->mfma_mn = 32; tile->lds_buffers = 2; tile->depth_u = mode_depth_u(plan->mode); tile->group_m = 1;
+    }
+    return FFGEMM_SUCCESS;
+}
+
+int ffgemm_grouped_plan_destroy(ffgemm_grouped_plan plan) {
+    if (!plan) return FFGEMM_SUCCESS;
+    if (plan->d_descs) hipFree(plan->d_descs);
+    delete plan;
+    return FFGEMM_SUCCESS;
+}
+
+}  // extern "C"

@@ -1,0 +1,465 @@
+// The tile engine: one MT_M x MT_N output tile of D = alpha*op(A)*op(B) + beta*C.
+//
+// Replaces the fetch/math wave pair of src/gemmMfmaKernelWriter.h
+// (fetch_loop :1226-1316, math_loop :1403-1447, fetch_c :1822-1906,
+// mfma_mfma_pang_with_store :1572-1768) with a single-role design: every wave
+// both loads (16-byte buffer loads into VGPRs, one K-tile ahead) and computes
+// (MFMA from a two-stage LDS ring), which on gfx950 keeps the MFMA pipe busy
+// without spending half the waves on address generation.
+#pragma once
+#include "gemm_common.hpp"
+
+namespace ffgemm {
+
+// ---------------------------------------------------------------------------
+// Operand loader: one MT x (128 bytes of K) tile of A (rows = M) or B (rows = N)
+// ---------------------------------------------------------------------------
+// KC: operand is K-contiguous in HBM (A of TN/TT, B of NN/TN).
+// Otherwise it is M/N-contiguous and each thread loads KB consecutive K-rows
+// of an 8 (16-bit) or 4 (32-bit) element column strip and transposes it in
+// registers.
+template <int MODE, bool KC, int MT, bool PERM>
+struct Loader {
+    static constexpr int ES = mode_elem_bytes(MODE);
+    static constexpr int DU = mode_depth_u(MODE);
+    static constexpr bool X3 = (MODE == kX3);
+    // loads of 16 bytes per thread
+    static constexpr int NLD = (MT * kRowBytes) / (16 * kThreads);
+    static_assert(NLD >= 1, "tile too small");
+    // M/N-contiguous: elements per 16-byte load along M/N, K-rows per thread
+    static constexpr int VE = 16 / ES;
+    static constexpr int KB = KC ? 0 : (DU * MT) / (VE * kThreads);
+    static_assert(KC || (KB == NLD && KB >= 2), "bad transposing geometry");
+    static_assert(KC || MODE != kF8, "fp8 operands must be K-contiguous");
+
+    __amdgpu_buffer_rsrc_t rsrc;
+    uint32_t voff[NLD];     // byte offset of each load for K-tile 0 (kOOB if masked)
+    int32_t kidx[NLD];      // K index of each load for K-tile 0
+    uint32_t kstep;         // bytes to advance per K-tile
+    int32_t K;
+    i32x4_t v[NLD];
+
+    // ld: leading dimension (elements); rows: M or N; row0: first row of tile
+    __device__ inline void init(const void* base, uint32_t extent_bytes, int32_t ld, int32_t rows,
+                                int32_t K_, int32_t row0) {
+        rsrc = make_rsrc(base, extent_bytes);
+        K = K_;
+        const int t = threadIdx.x;
+        if constexpr (KC) {
+            const int c = t & 7;
+#pragma unroll
+            for (int i = 0; i < NLD; ++i) {
+                int r = row0 + (t >> 3) + 32 * i;
+                int k = c * VE;
+                kidx[i] = k;
+                voff[i] = (r < rows) ? (uint32_t)(((int64_t)r * ld + k) * ES) : kOOB;
+            }
+            kstep = kRowBytes;
+        } else {
+            constexpr int NC = MT / VE;  // column strips across the tile
+            const int mc = t % NC, kg = t / NC;
+            const int r = row0 + mc * VE;
+#pragma unroll
+            for (int j = 0; j < KB; ++j) {
+                int k = kg * KB + j;
+                kidx[j] = k;
+                voff[j] = (r < rows) ? (uint32_t)(((int64_t)k * ld + r) * ES) : kOOB;
+            }
+            kstep = (uint32_t)(DU * ld * ES);
+        }
+    }
+
+    // Issue the 16-byte loads of K-tile kt.
+    __device__ inline void load(int kt) {
+        const int k0 = kt * DU;
+#pragma unroll
+        for (int i = 0; i < NLD; ++i) {
+            bool ok = (kidx[i] + k0) < K && voff[i] != kOOB;
+            uint32_t off = ok ? voff[i] + (uint32_t)kt * kstep : kOOB;
+            v[i] = buf_load16(rsrc, off);
+        }
+    }
+
+    // Write the loaded K-tile into one LDS stage (K-contiguous 128-byte rows).
+    __device__ inline void store(char* stage) const {
+        const int t = threadIdx.x;
+        if constexpr (KC) {
+            const int c = t & 7;
+#pragma unroll
+            for (int i = 0; i < NLD; ++i) {
+                int r = (t >> 3) + 32 * i;
+                int lr = PERM ? perm_a_row(r) : r;
+                if constexpr (!X3) {
+                    *(i32x4_t*)(stage + lds_addr(lr, c * 16)) = v[i];
+                } else {
+                    // 4 fp32 of K [4c, 4c+4) -> hi bf16 at bytes 8c, lo at 64 + 8c
+                    i32x2_t hi, lo;
+                    split4(v[i], hi, lo);
+                    *(i32x2_t*)(stage + lds_addr(lr, c * 8)) = hi;
+                    *(i32x2_t*)(stage + lds_addr(lr, 64 + c * 8)) = lo;
+                }
+            }
+        } else {
+            constexpr int NC = MT / VE;
+            const int mc = t % NC, kg = t / NC;
+            if constexpr (ES == 2) {
+                // KB rows of 8 x 16-bit -> 8 rows of KB x 16-bit
+#pragma unroll
+                for (int i = 0; i < 8; ++i) {
+                    int r = mc * 8 + i;
+                    int lr = PERM ? perm_a_row(r) : r;
+                    uint32_t w[KB / 2];
+#pragma unroll
+                    for (int p = 0; p < KB / 2; ++p) {
+                        uint32_t a = (uint32_t)v[2 * p][i >> 1];
+                        uint32_t b = (uint32_t)v[2 * p + 1][i >> 1];
+                        w[p] = (i & 1) ? ((a >> 16) | (b & 0xffff0000u)) : ((a & 0xffffu) | (b << 16));
+                    }
+                    char* dst = stage + lds_addr(lr, kg * KB * 2);
+                    if constexpr (KB == 8) *(i32x4_t*)dst = i32x4_t{(int)w[0], (int)w[1], (int)w[2], (int)w[3]};
+                    else if constexpr (KB == 4) *(i32x2_t*)dst = i32x2_t{(int)w[0], (int)w[1]};
+                    else *(int*)dst = (int)w[0];
+                }
+            } else {
+                // fp32: KB rows of 4 floats -> 4 rows of KB floats
+#pragma unroll
+                for (int i = 0; i < 4; ++i) {
+                    int r = mc * 4 + i;
+                    int lr = PERM ? perm_a_row(r) : r;
+                    if constexpr (!X3) {
+#pragma unroll
+                        for (int h = 0; h < KB / 4; ++h) {
+                            i32x4_t o = {v[4 * h][i], v[4 * h + 1][i], v[4 * h + 2][i], v[4 * h + 3][i]};
+                            *(i32x4_t*)(stage + lds_addr(lr, kg * KB * 4 + 16 * h)) = o;
+                        }
+                        if constexpr (KB == 2) {
+                            *(i32x2_t*)(stage + lds_addr(lr, kg * 8)) = i32x2_t{v[0][i], v[1][i]};
+                        }
+                    } else {
+                        uint32_t hw[KB / 2], lw[KB / 2];
+#pragma unroll
+                        for (int p = 0; p < KB / 2; ++p) {
+                            float a = __builtin_bit_cast(float, v[2 * p][i]);
+                            float b = __builtin_bit_cast(float, v[2 * p + 1][i]);
+                            split2(a, b, hw[p], lw[p]);
+                        }
+                        char* dh = stage + lds_addr(lr, kg * KB * 2);
+                        char* dl = stage + lds_addr(lr, 64 + kg * KB * 2);
+                        if constexpr (KB == 8) {
+                            *(i32x4_t*)dh = i32x4_t{(int)hw[0], (int)hw[1], (int)hw[2], (int)hw[3]};
+                            *(i32x4_t*)dl = i32x4_t{(int)lw[0], (int)lw[1], (int)lw[2], (int)lw[3]};
+                        } else if constexpr (KB == 4) {
+                            *(i32x2_t*)dh = i32x2_t{(int)hw[0], (int)hw[1]};
+                            *(i32x2_t*)dl = i32x2_t{(int)lw[0], (int)lw[1]};
+                        } else {
+                            *(int*)dh = (int)hw[0];
+                            *(int*)dl = (int)lw[0];
+                        }
+                    }
+                }
+            }
+        }
+    }
+
+    // bf16x3 split (Henry, Tang & Heinecke 2019): hi = rne(x), lo = rne(x - hi)
+    __device__ static inline void split2(float a, float b, uint32_t& hi, uint32_t& lo) {
+        __bf16 ha = (__bf16)a, hb = (__bf16)b;
+        __bf16 la = (__bf16)(a - (float)ha), lb = (__bf16)(b - (float)hb);
+        hi = (uint32_t)__builtin_bit_cast(uint16_t, ha) | ((uint32_t)__builtin_bit_cast(uint16_t, hb) << 16);
+        lo = (uint32_t)__builtin_bit_cast(uint16_t, la) | ((uint32_t)__builtin_bit_cast(uint16_t, lb) << 16);
+    }
+    __device__ static inline void split4(i32x4_t x, i32x2_t& hi, i32x2_t& lo) {
+        uint32_t h0, l0, h1, l1;
+        split2(__builtin_bit_cast(float, x[0]), __builtin_bit_cast(float, x[1]), h0, l0);
+        split2(__builtin_bit_cast(float, x[2]), __builtin_bit_cast(float, x[3]), h1, l1);
+        hi = i32x2_t{(int)h0, (int)h1};
+        lo = i32x2_t{(int)l0, (int)l1};
+    }
+};
+
+// ---------------------------------------------------------------------------
+// MFMA operand fragments and the per-read-step MFMA issue
+// ---------------------------------------------------------------------------
+template <int MODE>
+struct Frag;
+
+template <> struct Frag<kBF16> { i32x4_t x; };
+template <> struct Frag<kF16> { i32x4_t x; };
+template <> struct Frag<kF32> { i32x4_t x; };
+template <> struct Frag<kX3> { i32x4_t hi, lo; };
+template <> struct Frag<kF8> { i32x4_t x0, x1; };
+
+// read steps per K-tile
+template <int MODE>
+__host__ __device__ constexpr int read_steps() { return (MODE == kX3 || MODE == kF8) ? 2 : 4; }
+
+// Lane (row r of the operand stage, lane group g = lane/32) reads the chunks
+// that carry its K slice for read step rs.  A and B use the same assignment,
+// so every MFMA pairs matching K indices.
+template <int MODE>
+__device__ inline void read_frag(const char* stage, int row, int g, int rs, Frag<MODE>& f) {
+    if constexpr (MODE == kX3) {
+        f.hi = *(const i32x4_t*)(stage + lds_addr(row, (2 * rs + g) * 16));
+        f.lo = *(const i32x4_t*)(stage + lds_addr(row, (4 + 2 * rs + g) * 16));
+    } else if constexpr (MODE == kF8) {
+        f.x0 = *(const i32x4_t*)(stage + lds_addr(row, (4 * rs + g) * 16));
+        f.x1 = *(const i32x4_t*)(stage + lds_addr(row, (4 * rs + 2 + g) * 16));
+    } else {
+        f.x = *(const i32x4_t*)(stage + lds_addr(row, (2 * rs + g) * 16));
+    }
+}
+
+template <int MODE>
+__device__ inline void mfma(const Frag<MODE>& a, const Frag<MODE>& b, f32x16_t& acc) {
+    if constexpr (MODE == kBF16) {
+        acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(__builtin_bit_cast(bf16x8_t, a.x),
+                                                      __builtin_bit_cast(bf16x8_t, b.x), acc, 0, 0, 0);
+    } else if constexpr (MODE == kF16) {
+        acc = __builtin_amdgcn_mfma_f32_32x32x16_f16(__builtin_bit_cast(f16x8_t, a.x),
+                                                     __builtin_bit_cast(f16x8_t, b.x), acc, 0, 0, 0);
+    } else if constexpr (MODE == kF32) {
+#pragma unroll
+        for (int e = 0; e < 4; ++e)
+            acc = __builtin_amdgcn_mfma_f32_32x32x2f32(__builtin_bit_cast(float, a.x[e]),
+                                                       __builtin_bit_cast(float, b.x[e]), acc, 0, 0, 0);
+    } else if constexpr (MODE == kX3) {
+        // small cross terms first, then the dominant hi*hi product
+        acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(__builtin_bit_cast(bf16x8_t, a.lo),
+                                                      __builtin_bit_cast(bf16x8_t, b.hi), acc, 0, 0, 0);
+        acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(__builtin_bit_cast(bf16x8_t, a.hi),
+                                                      __builtin_bit_cast(bf16x8_t, b.lo), acc, 0, 0, 0);
+        acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(__builtin_bit_cast(bf16x8_t, a.hi),
+                                                      __builtin_bit_cast(bf16x8_t, b.hi), acc, 0, 0, 0);
+    } else {
+        i32x8_t av = {a.x0[0], a.x0[1], a.x0[2], a.x0[3], a.x1[0], a.x1[1], a.x1[2], a.x1[3]};
+        i32x8_t bv = {b.x0[0], b.x0[1], b.x0[2], b.x0[3], b.x1[0], b.x1[1], b.x1[2], b.x1[3]};
+        // format 0 = OCP e4m3 for both operands; e8m0 scale 127 = 1.0
+        acc = __builtin_amdgcn_mfma_scale_f32_32x32x64_f8f6f4(av, bv, acc, 0, 0, 0, 127, 0, 127);
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Epilogue: D = alpha*acc + beta*C (fp32 FMA), converted to the output type.
+// Lane l of MFMA tile (ti, tj) holds the 16 consecutive rows
+// 16*(l/32) + 0..15 of column (l%32).
+// ---------------------------------------------------------------------------
+template <int OUT>
+__device__ inline void store16(const TileArgs& a, const char* Cb, char* Db, int m, int n, const float* acc,
+                               float alpha, float beta) {
+    constexpr int OS = (OUT == kOutF32) ? 4 : 2;
+    const bool use_c = (a.flags & kFlagBeta) != 0;
+    char* dp = Db + ((int64_t)n * a.ldd + m) * OS;
+    const char* cp = Cb + ((int64_t)n * a.ldc + m) * OS;
+    if (m + 16 <= a.M) {
+        float r[16];
+        if (use_c) {
+            float c[16];
+            if constexpr (OUT == kOutF32) {
+#pragma unroll
+                for (int q = 0; q < 4; ++q) {
+                    f32x4_t cv = *(const f32x4_t*)(cp + 16 * q);
+                    c[4 * q] = cv[0]; c[4 * q + 1] = cv[1]; c[4 * q + 2] = cv[2]; c[4 * q + 3] = cv[3];
+                }
+            } else {
+#pragma unroll
+                for (int q = 0; q < 2; ++q) {
+                    i32x4_t cv = *(const i32x4_t*)(cp + 16 * q);
+#pragma unroll
+                    for (int e = 0; e < 4; ++e) {
+                        uint32_t u = (uint32_t)cv[e];
+                        c[8 * q + 2 * e] = (OUT == kOutBF16) ? bf16lo(u) : f16lo(u);
+                        c[8 * q + 2 * e + 1] = (OUT == kOutBF16) ? bf16hi(u) : f16hi(u);
+                    }
+                }
+            }
+#pragma unroll
+            for (int e = 0; e < 16; ++e) r[e] = __builtin_fmaf(alpha, acc[e], beta * c[e]);
+        } else {
+#pragma unroll
+            for (int e = 0; e < 16; ++e) r[e] = alpha * acc[e];
+        }
+        if constexpr (OUT == kOutF32) {
+#pragma unroll
+            for (int q = 0; q < 4; ++q)
+                *(f32x4_t*)(dp + 16 * q) = f32x4_t{r[4 * q], r[4 * q + 1], r[4 * q + 2], r[4 * q + 3]};
+        } else {
+            const bool trunc = (a.flags & kFlagTrunc) != 0;
+#pragma unroll
+            for (int q = 0; q < 2; ++q) {
+                uint32_t w[4];
+#pragma unroll
+                for (int e = 0; e < 4; ++e) {
+                    float x = r[8 * q + 2 * e], y = r[8 * q + 2 * e + 1];
+                    if constexpr (OUT == kOutBF16) w[e] = trunc ? pack_bf16x2_trunc(x, y) : pack_bf16x2_rne(x, y);
+                    else w[e] = pack_f16x2(x, y);
+                }
+                *(i32x4_t*)(dp + 16 * q) = i32x4_t{(int)w[0], (int)w[1], (int)w[2], (int)w[3]};
+            }
+        }
+    } else {
+        for (int e = 0; e < 16 && m + e < a.M; ++e) {
+            float c = 0.0f;
+            if (use_c) {
+                if constexpr (OUT == kOutF32) c = ((const float*)cp)[e];
+                else {
+                    uint16_t u = ((const uint16_t*)cp)[e];
+                    c = (OUT == kOutBF16) ? bf16lo(u) : f16lo(u);
+                }
+            }
+            float r = use_c ? __builtin_fmaf(alpha, acc[e], beta * c) : alpha * acc[e];
+            if constexpr (OUT == kOutF32) ((float*)dp)[e] = r;
+            else if constexpr (OUT == kOutBF16) {
+                uint32_t w = (a.flags & kFlagTrunc) ? pack_bf16x2_trunc(r, 0.f) : pack_bf16x2_rne(r, 0.f);
+                ((uint16_t*)dp)[e] = (uint16_t)(w & 0xffffu);
+            } else {
+                ((uint16_t*)dp)[e] = (uint16_t)(pack_f16x2(r, 0.f) & 0xffffu);
+            }
+        }
+    }
+}
+
+// Extent in bytes of one batch's operand as a buffer (clamped to 2 GiB - 16;
+// the host rejects problems that do not fit).
+__device__ inline uint32_t extent_bytes(int64_t rows, int64_t cols_ld, int64_t tail, int es) {
+    int64_t e = ((rows - 1) * cols_ld + tail) * es;
+    if (e < 0) e = 0;
+    if (e > 0x7ffffff0ll) e = 0x7ffffff0ll;
+    return (uint32_t)e;
+}
+
+// ---------------------------------------------------------------------------
+// One output tile.  smem: 2 stages x (MT_M + MT_N) x 128 bytes.
+// ---------------------------------------------------------------------------
+template <int MODE, bool AK, bool BK, int MT_M, int MT_N, int OUT>
+__device__ inline void gemm_tile(const TileArgs& a, int tm, int tn, int b, char* smem) {
+    constexpr int ES = mode_elem_bytes(MODE);
+    constexpr int DU = mode_depth_u(MODE);
+    constexpr int WT_M = MT_M / 2, WT_N = MT_N / 2;
+    constexpr int TM = WT_M / 32, TN = WT_N / 32;
+    constexpr int RS = read_steps<MODE>();
+    constexpr int STAGE = (MT_M + MT_N) * kRowBytes;
+    static_assert(TM >= 1 && TN >= 1, "wave tile below one MFMA tile");
+
+    const int lane = threadIdx.x & 63;
+    const int wave = threadIdx.x >> 6;
+    const int wm = wave & 1, wn = wave >> 1;
+    const int m0 = tm * MT_M, n0 = tn * MT_N;
+
+    const char* Ab = (const char*)a.A + (int64_t)b * a.bsA * ES;
+    const char* Bb = (const char*)a.B + (int64_t)b * a.bsB * ES;
+
+    Loader<MODE, AK, MT_M, true> la;
+    Loader<MODE, BK, MT_N, false> lb;
+    // A: K-contiguous = rows M of length lda; M-contiguous = rows K of length lda
+    la.init(Ab, AK ? extent_bytes(a.M, a.lda, a.K, ES) : extent_bytes(a.K, a.lda, a.M, ES), a.lda, a.M, a.K, m0);
+    lb.init(Bb, BK ? extent_bytes(a.N, a.ldb, a.K, ES) : extent_bytes(a.K, a.ldb, a.N, ES), a.ldb, a.N, a.K, n0);
+
+    f32x16_t acc[TM][TN];
+#pragma unroll
+    for (int i = 0; i < TM; ++i)
+#pragma unroll
+        for (int j = 0; j < TN; ++j)
+#pragma unroll
+            for (int e = 0; e < 16; ++e) acc[i][j][e] = 0.0f;
+
+    const int KT = (a.K + DU - 1) / DU;
+    const int g = lane >> 5;
+    const int arow = wm * WT_M + (lane & 31);
+    const int brow = wn * WT_N + (lane & 31);
+
+    if (KT > 0) {
+        la.load(0);
+        lb.load(0);
+        la.store(smem);
+        lb.store(smem + MT_M * kRowBytes);
+        lds_barrier();
+    }
+
+    for (int kt = 0; kt < KT; ++kt) {
+        const char* sa = smem + (kt & 1) * STAGE;
+        const char* sb = sa + MT_M * kRowBytes;
+        const bool more = (kt + 1) < KT;
+        if (more) {
+            la.load(kt + 1);
+            lb.load(kt + 1);
+        }
+        Frag<MODE> fa[2][TM], fb[2][TN];
+#pragma unroll
+        for (int i = 0; i < TM; ++i) read_frag<MODE>(sa, arow + 32 * i, g, 0, fa[0][i]);
+#pragma unroll
+        for (int j = 0; j < TN; ++j) read_frag<MODE>(sb, brow + 32 * j, g, 0, fb[0][j]);
+#pragma unroll
+        for (int rs = 0; rs < RS; ++rs) {
+            if (rs + 1 < RS) {
+#pragma unroll
+                for (int i = 0; i < TM; ++i) read_frag<MODE>(sa, arow + 32 * i, g, rs + 1, fa[(rs + 1) & 1][i]);
+#pragma unroll
+                for (int j = 0; j < TN; ++j) read_frag<MODE>(sb, brow + 32 * j, g, rs + 1, fb[(rs + 1) & 1][j]);
+            }
+#pragma unroll
+            for (int i = 0; i < TM; ++i)
+#pragma unroll
+                for (int j = 0; j < TN; ++j) mfma<MODE>(fa[rs & 1][i], fb[rs & 1][j], acc[i][j]);
+        }
+        if (more) {
+            char* na = smem + ((kt + 1) & 1) * STAGE;
+            la.store(na);
+            lb.store(na + MT_M * kRowBytes);
+        }
+        lds_barrier();
+    }
+
+    // epilogue
+    float alpha = a.alpha;
+    if constexpr (MODE == kF8) {
+        float sa = a.scaleA ? *a.scaleA : 1.0f;
+        float sb = a.scaleB ? *a.scaleB : 1.0f;
+        alpha = alpha / (sa * sb);
+    }
+    const char* Cb = (const char*)a.C + (int64_t)b * a.bsC * ((OUT == kOutF32) ? 4 : 2);
+    char* Db = (char*)a.D + (int64_t)b * a.bsD * ((OUT == kOutF32) ? 4 : 2);
+#pragma unroll
+    for (int i = 0; i < TM; ++i) {
+        const int m = m0 + wm * WT_M + 32 * i + 16 * g;
+#pragma unroll
+        for (int j = 0; j < TN; ++j) {
+            const int n = n0 + wn * WT_N + 32 * j + (lane & 31);
+            if (n < a.N && m < a.M) {
+                float v[16];
+#pragma unroll
+                for (int e = 0; e < 16; ++e) v[e] = acc[i][j][e];
+                store16<OUT>(a, Cb, Db, m, n, v, alpha, a.beta);
+            }
+        }
+    }
+}
+
+// Tile index -> (batch, tile_m, tile_n).  Workgroups are dispatched to the 8
+// XCDs round-robin (block b runs on XCD b % 8); the remap gives each XCD a
+// contiguous range of the grouped raster so its L2 sees a compact block of
+// A rows and B columns.
+__device__ inline void map_tile(int bid, int nblocks, const TileArgs& a, int& tb, int& tm, int& tn) {
+    int per = nblocks >> 3;
+    int v = (bid < (per << 3)) ? ((bid & 7) * per + (bid >> 3)) : bid;
+    const int tiles = a.tiles_m * a.tiles_n;
+    tb = v / tiles;
+    int t = v - tb * tiles;
+    const int gm = a.group_m > 0 ? a.group_m : 1;
+    const int gsize = gm * a.tiles_n;
+    const int grp = t / gsize;
+    const int first = grp * gm;
+    const int h = min(a.tiles_m - first, gm);
+    const int r = t - grp * gsize;
+    tm = first + r % h;
+    tn = r / h;
+}
+
+template <int MODE, bool AK, bool BK, int MT_M, int MT_N, int OUT>
+__global__ void __launch_bounds__(kThreads, 1) gemm_kernel(TileArgs a) {
+    __shared__ __attribute__((aligned(16))) char smem[2 * (MT_M + MT_N) * kRowBytes];
+    int tb, tm, tn;
+    map_tile(blockIdx.x, gridDim.x, a, tb, tm, tn);
+    gemm_tile<MODE, AK, BK, MT_M, MT_N, OUT>(a, tm, tn, tb, smem);
+}
+
+}  // namespace ffgemm

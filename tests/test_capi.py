@@ -1,0 +1,97 @@
+"""C ABI checks that need no GPU: the library loads, exports every entry point
+include/ffgemm.h declares, and rejects what checkKernelParameters would
+(src/gemmMfmaKernelWriter.h:392-399) with RTN_ERR instead of exiting."""
+from __future__ import annotations
+
+import ctypes
+import os
+import re
+
+import pytest
+
+import tensilelite_amd as tl
+from tensilelite_amd import _capi as C
+
+HEADER = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "include", "ffgemm.h")
+
+
+def declared_functions():
+    src = open(HEADER).read()
+    src = re.sub(r"/\*.*?\*/", "", src, flags=re.S)
+    return sorted(set(re.findall(r"\b(ffgemm_[a-z0-9_]+)\s*\(", src)))
+
+
+def test_library_exports_every_declared_symbol():
+    lib = C.lib()
+    names = declared_functions()
+    assert len(names) >= 15
+    missing = [n for n in names if not hasattr(lib, n)]
+    assert not missing, missing
+
+
+def test_status_strings_and_version():
+    lib = C.lib()
+    assert [lib.ffgemm_status_string(i).decode() for i in range(4)] == ["SUCCESS", "RTN_WARN", "RTN_ERR", "RTN_FATAL"]
+    assert b"gfx950" in lib.ffgemm_version()
+
+
+def _p(layout="NN", dtype=C.BF16, M=512, N=512, K=512, **kw):
+    ta, tb = layout[0] == "T", layout[1] == "T"
+    d = dict(lda=K if ta else M, ldb=N if tb else K, ldc=M, ldd=M)
+    d.update(kw)
+    return tl.make_problem(layout, dtype, M, N, K, d["lda"], d["ldb"], d["ldc"], d["ldd"],
+                           alpha=kw.get("alpha", 1.0), beta=kw.get("beta", 0.0), compute=kw.get("compute", 0),
+                           dtype_out=kw.get("dtype_out", 0))
+
+
+@pytest.mark.parametrize("layout", ["NN", "NT", "TN", "TT"])
+@pytest.mark.parametrize("dtype", [C.BF16, C.F16, C.F32])
+def test_query_tile_accepts_supported(layout, dtype):
+    t = tl.query_tile(_p(layout, dtype, 4096, 4096, 4096))
+    assert (t.mt_m, t.mt_n) == (256, 256)
+    assert t.waves_m == 2 and t.waves_n == 2 and t.mfma_mn == 32
+    assert t.depth_u == (64 if dtype != C.F32 else 32)
+
+
+def test_query_tile_small_shapes_pick_small_tiles():
+    t = tl.query_tile(_p("NN", C.BF16, 480, 512, 512))
+    assert (t.mt_m, t.mt_n) == (64, 64)
+
+
+@pytest.mark.parametrize("bad, msg", [
+    (dict(K=500, layout="TN", lda=504, ldb=504), "multiple of 8"),   # K-contiguous operand, K % 8
+    (dict(lda=100), "lda"),                                # lda < M
+    (dict(ldd=516), "multiples"),                          # 16-byte rows of D
+    (dict(layout="NN", dtype=C.F8E4M3), "TN"),             # fp8 is TN only
+    (dict(compute=C.COMPUTE_BF16X3), "fp32 operands"),     # bf16x3 needs fp32
+    (dict(dtype=9), "dtype"),
+])
+def test_query_tile_rejects_with_rtn_err(bad, msg):
+    kw = dict(bad)
+    layout = kw.pop("layout", "NN")
+    dtype = kw.pop("dtype", C.BF16)
+    K = kw.pop("K", 512)
+    p = _p(layout, dtype, 512, 512, K, **kw)
+    with pytest.raises(tl.FFGemmError) as e:
+        tl.query_tile(p)
+    assert e.value.status == C.ERR
+    assert msg in str(e.value)
+
+
+def test_bad_tile_request_is_rejected():
+    with pytest.raises(tl.FFGemmError) as e:
+        tl.query_tile(_p(), tile=(96, 96))
+    assert "unsupported macro tile" in str(e.value)
+
+
+def test_kernarg_entry_rejects_bad_dtype_without_launch():
+    st = C.lib().ffgemm_run_tn_kernarg(None, None, None, None, 1.0, 0.0, 64, 0, 64, 0, 64, 0, 64, 0,
+                                       64, 64, 64, 1, 7, None, None)
+    assert st == C.ERR
+    assert b"data type" in C.lib().ffgemm_last_error()
+
+
+def test_grouped_plan_rejects_empty_group():
+    h = ctypes.c_void_p()
+    st = C.lib().ffgemm_grouped_plan_create(ctypes.byref(h), 0, None, None, None, None, None, None)
+    assert st == C.ERR

@@ -1,0 +1,322 @@
+"""GPU parity of the gemmMfma path (through the C ABI) against the oracle.
+
+Tolerances (stated per BASELINE north star):
+  fp16/bf16 inputs: per element |d - r| <= 1e-2 * (|r| + rms(r)) vs the FP64
+      oracle on the same (rounded) inputs;
+  fp32 (fp32 MFMA) and bf16x3: relative Frobenius error <= 1e-5 vs FP64;
+  fp8 e4m3: quantised bytes bit-exact vs the oracle's e4m3 encoder, GEMM
+      relative Frobenius <= 1e-5 vs FP64 on the dequantised operands;
+  fp64: relative Frobenius <= 1e-12 vs numpy float64.
+"""
+from __future__ import annotations
+
+import sys
+
+import numpy as np
+import pytest
+import torch
+
+import oracle
+import tensilelite_amd as tl
+from tensilelite_amd import _capi as C
+from tests._util import bits_to_f32, colmajor, elem_ok, extract, op_dims, quantize_bits, rel_fro
+
+pytestmark = pytest.mark.gpu
+
+TILES = [(256, 256), (256, 128), (128, 256), (128, 128), (64, 64)]
+LAYOUTS = ["NN", "NT", "TN", "TT"]
+
+
+def _say(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+def _dev16(bits, name, device):
+    t = torch.from_numpy(bits.view(np.int16).copy()).to(device)
+    return t.view(torch.bfloat16 if name == "bf16" else torch.float16)
+
+
+def _f64_ref(layout, M, N, K, A, lda, B, ldb, alpha=1.0, beta=0.0, Cm=None, ldc=None, ldd=None,
+             batch=1, bsA=0, bsB=0, bsC=0, bsD=0):
+    return oracle.gemm_f64(layout, M, N, K, alpha, A.astype(np.float64), lda, B.astype(np.float64), ldb,
+                           beta, None if Cm is None else Cm.astype(np.float64), ldc, ldd, batch, bsA, bsB,
+                           bsC, bsD)
+
+
+@pytest.mark.parametrize("name", ["bf16", "f16"])
+@pytest.mark.parametrize("layout", LAYOUTS)
+@pytest.mark.parametrize("tile", TILES)
+def test_16bit_layouts_tiles_edges(cuda, name, layout, tile):
+    # M, N not multiples of any tile; K = 200 leaves a partial K-tile of 8.
+    M, N, K = 328, 272, 200
+    _say(f"case {name} {layout} tile={tile} M={M} N={N} K={K}")
+    rng = np.random.default_rng(11)
+    (ra, ca), (rb, cb) = op_dims(layout, M, N, K)
+    A = colmajor(rng, ra, ca, kind="normal", scale=1 / np.sqrt(K))
+    B = colmajor(rng, rb, cb, kind="normal", scale=1 / np.sqrt(K))
+    Ab, Bb = quantize_bits(A, name), quantize_bits(B, name)
+    Ar, Br = bits_to_f32(Ab, name), bits_to_f32(Bb, name)
+    D = tl.gemm(layout, M, N, K, _dev16(Ab, name, cuda), _dev16(Bb, name, cuda), tile=tile)
+    torch.cuda.synchronize()
+    got = D.float().cpu().numpy()
+    ref = _f64_ref(layout, M, N, K, Ar, ra, Br, rb)
+    bad, worst = elem_ok(got, ref)
+    assert bad == 0, f"{bad} elements off (worst scaled error {worst:.3g})"
+
+
+@pytest.mark.parametrize("layout", LAYOUTS)
+@pytest.mark.parametrize("tile", [(256, 256), (128, 128), (64, 64)])
+@pytest.mark.parametrize("compute", ["f32", "bf16x3"])
+def test_fp32_accuracy_forms(cuda, layout, tile, compute):
+    M, N, K = 200, 264, 132
+    _say(f"case {compute} {layout} tile={tile}")
+    rng = np.random.default_rng(2)
+    (ra, ca), (rb, cb) = op_dims(layout, M, N, K)
+    A = colmajor(rng, ra, ca, kind="normal")
+    B = colmajor(rng, rb, cb, kind="normal")
+    comp = C.COMPUTE_BF16X3 if compute == "bf16x3" else C.COMPUTE_DEFAULT
+    D = tl.gemm(layout, M, N, K, torch.from_numpy(A).to(cuda), torch.from_numpy(B).to(cuda),
+                compute=comp, tile=tile)
+    torch.cuda.synchronize()
+    err = rel_fro(D.cpu().numpy(), _f64_ref(layout, M, N, K, A, ra, B, rb))
+    _say(f"rel_fro={err:.3e}")
+    assert err <= 1e-5
+
+
+@pytest.mark.parametrize("name", ["bf16", "f16", "f32"])
+def test_alpha_beta_padding(cuda, name):
+    M, N, K = 256, 192, 128
+    lda, ldb, ldc, ldd = K + 32, K + 64, M + 16, M + 8  # TN with padded rows, as the reference
+    _say(f"case alpha/beta {name}")
+    rng = np.random.default_rng(5)
+    A = colmajor(rng, K, M, lda)
+    B = colmajor(rng, K, N, ldb)
+    Cm = colmajor(rng, M, N, ldc)
+    Dinit = np.full(ldd * N, 7.0, dtype=np.float32)
+    if name == "f32":
+        dev = lambda x: torch.from_numpy(x).to(cuda)
+        Ar, Br, Cr = A, B, Cm
+    else:
+        bits = [quantize_bits(x, name) for x in (A, B, Cm, Dinit)]
+        dev = None
+        Ar, Br, Cr = (bits_to_f32(b, name) for b in bits[:3])
+    alpha, beta = 0.314, 0.2718
+    if name == "f32":
+        Ad, Bd, Cd, Dd = dev(A), dev(B), dev(Cm), dev(Dinit)
+    else:
+        Ad, Bd, Cd, Dd = (_dev16(b, name, cuda) for b in bits)
+    tl.gemm("TN", M, N, K, Ad, Bd, Cd, Dd, lda=lda, ldb=ldb, ldc=ldc, ldd=ldd, alpha=alpha, beta=beta)
+    torch.cuda.synchronize()
+    got = Dd.float().cpu().numpy()
+    ref = _f64_ref("TN", M, N, K, Ar, lda, Br, ldb, alpha, beta, Cr, ldc, ldd)
+    g, r = extract(got, M, N, ldd), extract(ref, M, N, ldd)
+    if name == "f32":
+        assert rel_fro(g, r) <= 1e-5
+    else:
+        bad, worst = elem_ok(g, r)
+        assert bad == 0, worst
+    # padding rows of D are untouched
+    pad = got.reshape(N, ldd)[:, M:]
+    assert np.all(pad == 7.0)
+
+
+def test_beta_zero_never_reads_c(cuda):
+    M, N, K = 128, 128, 64
+    _say("case beta0 NaN C")
+    rng = np.random.default_rng(6)
+    A = colmajor(rng, M, K)
+    B = colmajor(rng, K, N)
+    Cm = torch.full((M * N,), float("nan"), device=cuda)
+    D = tl.gemm("NN", M, N, K, torch.from_numpy(A).to(cuda), torch.from_numpy(B).to(cuda), Cm, beta=0.0)
+    torch.cuda.synchronize()
+    assert torch.isfinite(D).all()
+
+
+def test_k_zero_and_alpha_zero(cuda):
+    M, N = 96, 80
+    _say("case K=0 / alpha=0")
+    rng = np.random.default_rng(7)
+    Cm = colmajor(rng, M, N)
+    Cd = torch.from_numpy(Cm).to(cuda)
+    A = torch.zeros(M * 8, device=cuda)
+    B = torch.zeros(8 * N, device=cuda)
+    D = tl.gemm("NN", M, N, 0, A, B, Cd, lda=M, ldb=8, alpha=1.0, beta=0.5)
+    torch.cuda.synchronize()
+    np.testing.assert_array_equal(D.cpu().numpy(), (0.5 * Cm).astype(np.float32))
+    A = torch.from_numpy(colmajor(rng, M, 64)).to(cuda)
+    B = torch.from_numpy(colmajor(rng, 64, N)).to(cuda)
+    D = tl.gemm("NN", M, N, 64, A, B, Cd, alpha=0.0, beta=1.0)
+    torch.cuda.synchronize()
+    np.testing.assert_array_equal(D.cpu().numpy(), Cm)
+
+
+def test_zero_sized_is_noop(cuda):
+    D = torch.full((16,), 3.0, device=cuda)
+    tl.gemm("NN", 0, 4, 8, torch.zeros(64, device=cuda), torch.zeros(32, device=cuda), D=D, lda=8, ldd=4)
+    torch.cuda.synchronize()
+    assert torch.all(D == 3.0)
+
+
+@pytest.mark.parametrize("name", ["bf16", "f32"])
+def test_batched(cuda, name):
+    M, N, K, batch = 136, 120, 96, 3
+    _say(f"case batch {name}")
+    rng = np.random.default_rng(8)
+    lda, ldb, ldd = K, K, M
+    bsA, bsB, bsD = lda * M + 64, ldb * N + 128, ldd * N + 32
+    A = np.concatenate([np.pad(colmajor(rng, K, M, lda), (0, 64)) for _ in range(batch)])
+    B = np.concatenate([np.pad(colmajor(rng, K, N, ldb), (0, 128)) for _ in range(batch)])
+    if name == "f32":
+        Ad, Bd, Ar, Br = torch.from_numpy(A).to(cuda), torch.from_numpy(B).to(cuda), A, B
+    else:
+        Ab, Bb = quantize_bits(A, name), quantize_bits(B, name)
+        Ad, Bd, Ar, Br = _dev16(Ab, name, cuda), _dev16(Bb, name, cuda), bits_to_f32(Ab, name), bits_to_f32(Bb, name)
+    D = tl.gemm("TN", M, N, K, Ad, Bd, batch=batch, bsA=bsA, bsB=bsB, bsD=bsD, lda=lda, ldb=ldb, ldd=ldd,
+                D=torch.zeros(bsD * batch, dtype=Ad.dtype, device=cuda))
+    torch.cuda.synchronize()
+    ref = _f64_ref("TN", M, N, K, Ar, lda, Br, ldb, ldd=ldd, batch=batch, bsA=bsA, bsB=bsB, bsD=bsD)
+    g = extract(D.float().cpu().numpy(), M, N, ldd, batch, bsD)
+    r = extract(ref, M, N, ldd, batch, bsD)
+    if name == "f32":
+        assert rel_fro(g, r) <= 1e-5
+    else:
+        assert elem_ok(g, r)[0] == 0
+
+
+@pytest.mark.parametrize("dtype", [1, 2, 3])
+def test_reference_kat_tn_kernarg(cuda, dtype):
+    """The reference's own run: InitRealData patterns (A = idx%7, B = -(idx%11),
+    C = idx%7), alpha 0.314, beta 0.2718, padding 32, TN, through the
+    72-byte-kernarg-shaped entry point (src/gemmMfma.cpp:106-119, 210-235)."""
+    M, N, K, pad = 480, 512, 512, 32
+    _say(f"case KAT dtype={dtype}")
+    lda = ldb = K + pad
+    ldc = ldd = M + pad
+    A = oracle.init_real_data(2.4, lda, M)
+    B = oracle.init_real_data(-2.4, ldb, N)
+    Cm = oracle.init_real_data(2.4, ldc, N)
+    alpha, beta = 0.314, 0.2718
+    ref = oracle.gemm_f32("TN", M, N, K, alpha, A, lda, B, ldb, beta, Cm, ldc, ldd)
+    if dtype == 1:
+        dev = [torch.from_numpy(x).to(cuda) for x in (A, B, Cm)]
+        D = torch.zeros(ldd * N, device=cuda)
+    else:
+        name = "f16" if dtype == 2 else "bf16"
+        # the reference converts bf16 by truncation (inc/ff_utils.h:119-122); integers are exact either way
+        dev = [_dev16(quantize_bits(x, name, truncate=True), name, cuda) for x in (A, B, Cm)]
+        D = torch.zeros(ldd * N, dtype=dev[0].dtype, device=cuda)
+    st = C.lib().ffgemm_run_tn_kernarg(D.data_ptr(), dev[2].data_ptr(), dev[0].data_ptr(), dev[1].data_ptr(),
+                                       alpha, beta, ldd, ldd * N, ldc, ldc * N, lda, lda * M, ldb, ldb * N,
+                                       M, N, K, 1, dtype, None, None)
+    C.check(st)
+    torch.cuda.synchronize()
+    g = extract(D.float().cpu().numpy(), M, N, ldd)
+    r = extract(ref, M, N, ldd)
+    if dtype == 1:
+        # the reference's own rule: |gpu - cpu| <= 0.1 (src/gemmMfma.cpp:401)
+        assert np.max(np.abs(g - r)) <= 0.1
+    else:
+        bad, worst = elem_ok(g, r)
+        assert bad == 0, worst
+
+
+def test_bf16_truncate_matches_reference_store(cuda):
+    """bf16 D rounded by truncation, as the reference's store
+    (src/gemmMfmaKernelWriter.h:1661-1664).  Integer data keeps every fp32 sum
+    exact, so D must equal trunc(fma(alpha, sum, beta*c)) to the bit."""
+    M, N, K = 128, 128, 128
+    _say("case bf16 truncate")
+    rng = np.random.default_rng(9)
+    A = colmajor(rng, K, M, kind="int")
+    B = colmajor(rng, K, N, kind="int")
+    Cm = colmajor(rng, M, N, kind="int")
+    name = "bf16"
+    Ad, Bd, Cd = (_dev16(quantize_bits(x, name), name, cuda) for x in (A, B, Cm))
+    alpha, beta = 0.5, 0.25  # powers of two keep alpha*sum + beta*c exact
+    D = tl.gemm("TN", M, N, K, Ad, Bd, Cd, alpha=alpha, beta=beta, round=C.ROUND_TRUNCATE)
+    torch.cuda.synchronize()
+    exact = oracle.gemm_f64("TN", M, N, K, alpha, A.astype(np.float64), K, B.astype(np.float64), K, beta,
+                            Cm.astype(np.float64), M, M).astype(np.float32)
+    want = oracle.f32_to_bf16_bits(exact, truncate=True)
+    got = D.view(torch.int16).cpu().numpy().view(np.uint16)
+    np.testing.assert_array_equal(got, want)
+
+
+def test_fp8_quantize_bit_exact_and_gemm(cuda):
+    M, N, K = 192, 160, 256
+    _say("case fp8")
+    rng = np.random.default_rng(2)
+    A = colmajor(rng, M, K, kind="normal")   # NN-stored inputs: A M-contiguous, B K-contiguous
+    B = colmajor(rng, K, N, kind="normal")
+    Ad, Bd = torch.from_numpy(A).to(cuda), torch.from_numpy(B).to(cuda)
+    qa, sa = tl.fp8.quantize(Ad, rows=M, k=K, ldx=M, trans_x=False)
+    qb, sb = tl.fp8.quantize(Bd, rows=N, k=K, ldx=K, trans_x=True)
+    torch.cuda.synchronize()
+    sa_ref = oracle.fp8_scale(np.abs(A).max())
+    sb_ref = oracle.fp8_scale(np.abs(B).max())
+    assert sa.item() == np.float32(sa_ref) and sb.item() == np.float32(sb_ref)
+    # oracle encoding of the same operands in the K-contiguous layout
+    At = A.reshape(K, M).T.copy().ravel()   # (m, k) at k + m*K
+    qa_ref = oracle.f32_to_e4m3_bits(At, np.float32(sa_ref))
+    qb_ref = oracle.f32_to_e4m3_bits(B, np.float32(sb_ref))
+    np.testing.assert_array_equal(qa.cpu().numpy(), qa_ref)
+    np.testing.assert_array_equal(qb.cpu().numpy(), qb_ref)
+    D = tl.gemm("TN", M, N, K, qa, qb, lda=K, ldb=K, scaleA=sa, scaleB=sb, dtype_out=torch.float32)
+    torch.cuda.synchronize()
+    deq_a = oracle.e4m3_bits_to_f32(qa_ref).astype(np.float64) / sa_ref
+    deq_b = oracle.e4m3_bits_to_f32(qb_ref).astype(np.float64) / sb_ref
+    ref = oracle.gemm_f64("TN", M, N, K, 1.0, deq_a, K, deq_b, K)
+    err = rel_fro(D.cpu().numpy(), ref)
+    err_vs_fp32_inputs = rel_fro(D.cpu().numpy(), _f64_ref("NN", M, N, K, A, M, B, K))
+    _say(f"fp8 rel_fro vs dequantised {err:.3e}; vs fp32 inputs {err_vs_fp32_inputs:.3e}")
+    assert err <= 1e-5
+    assert err_vs_fp32_inputs < 0.1
+
+
+@pytest.mark.parametrize("layout", LAYOUTS)
+def test_fp64_oracle_product(cuda, layout):
+    M, N, K = 130, 70, 90
+    _say(f"case f64 {layout}")
+    rng = np.random.default_rng(4)
+    (ra, ca), (rb, cb) = op_dims(layout, M, N, K)
+    A = rng.uniform(-1, 1, ra * ca)
+    B = rng.uniform(-1, 1, rb * cb)
+    Cm = rng.uniform(-1, 1, M * N)
+    D = tl.gemm(layout, M, N, K, torch.from_numpy(A).to(cuda), torch.from_numpy(B).to(cuda),
+                torch.from_numpy(Cm).to(cuda), alpha=0.75, beta=-0.5)
+    torch.cuda.synchronize()
+    ref = oracle.gemm_f64(layout, M, N, K, 0.75, A, ra, B, rb, -0.5, Cm, M, M)
+    assert rel_fro(D.cpu().numpy(), ref) <= 1e-12
+
+
+@pytest.mark.parametrize("layout", ["NN", "NT", "TN"])
+@pytest.mark.parametrize("beta", [0.0, 1.0])
+def test_4096_bf16_sampled_columns(cuda, layout, beta):
+    """Config 2 at full size: 24 random output columns against the FP64 oracle
+    and every column sum against (1^T op(A)) op(B) in FP64."""
+    M = N = K = 4096
+    _say(f"case 4096 {layout} beta={beta}")
+    g = torch.Generator(device="cpu").manual_seed(1)
+    (ra, ca), (rb, cb) = op_dims(layout, M, N, K)
+    A32 = (torch.randn(ra * ca, generator=g) / np.sqrt(K)).to(torch.bfloat16)
+    B32 = (torch.randn(rb * cb, generator=g) / np.sqrt(K)).to(torch.bfloat16)
+    C32 = (torch.rand(M * N, generator=g) * 2 - 1).to(torch.bfloat16)
+    D = tl.gemm(layout, M, N, K, A32.to(cuda), B32.to(cuda), C32.to(cuda), beta=beta)
+    torch.cuda.synchronize()
+    Dh = D.float().cpu().numpy().reshape(N, M)
+    A64 = A32.double().numpy()
+    B64 = B32.double().numpy()
+    C64 = C32.double().numpy().reshape(N, M)
+    rng = np.random.default_rng(0)
+    cols = rng.choice(N, 24, replace=False)
+    opA = A64.reshape(ca, ra).T if layout[0] == "N" else A64.reshape(ca, ra)      # (M, K)
+    opB = B64.reshape(cb, rb).T if layout[1] == "N" else B64.reshape(cb, rb)      # (K, N)
+    sub = np.ascontiguousarray(opB[:, cols])                                     # (K, 24)
+    ref_cols = oracle.gemm_f64("TN", M, len(cols), K, 1.0, np.ascontiguousarray(opA).ravel(), K,
+                               sub.T.ravel(), K).reshape(len(cols), M).T + beta * C64[cols].T
+    bad, worst = elem_ok(Dh[cols].T, ref_cols)
+    assert bad == 0, worst
+    colsum_ref = (opA.sum(axis=0) @ opB) + beta * C64.sum(axis=1)
+    colsum = Dh.astype(np.float64).sum(axis=1)
+    assert rel_fro(colsum, colsum_ref) < 2e-3
