@@ -66,6 +66,14 @@ def main():
 
     dev = torch.device("cuda:0")
     torch.cuda.set_device(dev)
+    if a.only == "copy":
+        # calibration for HBM byte counters: a plain device copy of 256 MiB
+        x = torch.ones(64 << 20, device=dev)
+        y = torch.empty_like(x)
+        ms = time_launches(lambda: y.copy_(x), a.steps, a.warmup, torch)
+        print(json.dumps({"copy_bytes_read": x.numel() * 4, "copy_bytes_written": y.numel() * 4,
+                          "ms": ms, "GBps": 2 * x.numel() * 4 / (ms * 1e-3) / 1e9}))
+        return
     M = N = K = 4096
     flop = 2.0 * M * N * K
     g = torch.Generator(device="cpu").manual_seed(1)

@@ -121,8 +121,8 @@ __global__ void __launch_bounds__(256) gemm_f64_kernel(F64Args a) {
     for (int i = 0; i < 2; ++i)
         for (int j = 0; j < 2; ++j)
             for (int e = 0; e < 4; ++e) {
-                // output row 4*(l/16) + e, column l%16 of the 16 x 16 tile
-                int64_t m = m0 + wm * 32 + i * 16 + 4 * (lane >> 4) + e;
+                // output row l/16 + 4*e, column l%16 of the 16 x 16 tile
+                int64_t m = m0 + wm * 32 + i * 16 + (lane >> 4) + 4 * e;
                 int64_t n = n0 + wn * 32 + j * 16 + (lane & 15);
                 if (m < a.M && n < a.N) {
                     double r = a.alpha * acc[i][j][e];

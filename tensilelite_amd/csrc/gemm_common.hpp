@@ -96,6 +96,10 @@ __device__ inline uint32_t pack_f16x2(float a, float b) {
     _Float16 x = (_Float16)a, y = (_Float16)b;
     return (uint32_t)__builtin_bit_cast(uint16_t, x) | ((uint32_t)__builtin_bit_cast(uint16_t, y) << 16);
 }
+// Reinterpret an int taken by value.  (A bit_cast applied directly to a
+// vector-element lvalue such as v[i] reads element 0 with this compiler.)
+__device__ inline float as_float(int v) { return __builtin_bit_cast(float, v); }
+
 __device__ inline float bf16lo(uint32_t v) { return __builtin_bit_cast(float, v << 16); }
 __device__ inline float bf16hi(uint32_t v) { return __builtin_bit_cast(float, v & 0xffff0000u); }
 __device__ inline float f16lo(uint32_t v) { return (float)__builtin_bit_cast(_Float16, (uint16_t)(v & 0xffffu)); }

@@ -2,7 +2,7 @@
 // instantiates the (layout x tile) table for its mode, so the heavy template
 // compiles in parallel translation units.
 #pragma once
-#include "gemm_tile.hpp"
+#include "gemm_tile_v2.hpp"
 
 namespace ffgemm {
 
@@ -11,9 +11,14 @@ namespace ffgemm {
 
 template <int MODE, bool AK, bool BK, int OUT>
 inline int launch_layout(int mt_m, int mt_n, const TileArgs& a, int grid, hipStream_t s) {
+    // bf16/fp16 tiles of 128 and up run the deep-pipelined engine (v2::)
+    constexpr bool k16 = (MODE == kBF16 || MODE == kF16);
 #define FFGEMM_CASE(TM_, TN_)                                                                  \
     if (mt_m == TM_ && mt_n == TN_) {                                                          \
-        hipLaunchKernelGGL((gemm_kernel<MODE, AK, BK, TM_, TN_, OUT>), dim3(grid), dim3(kThreads), 0, s, a); \
+        if constexpr (k16 && TM_ >= 128 && TN_ >= 128)                                         \
+            hipLaunchKernelGGL((v2::gemm_kernel<MODE, AK, BK, TM_, TN_, OUT>), dim3(grid), dim3(kThreads), 0, s, a); \
+        else                                                                                   \
+            hipLaunchKernelGGL((gemm_kernel<MODE, AK, BK, TM_, TN_, OUT>), dim3(grid), dim3(kThreads), 0, s, a); \
         return 0;                                                                              \
     }
     FFGEMM_TILES(FFGEMM_CASE)

@@ -31,6 +31,9 @@ struct Loader {
     static constexpr int KB = KC ? 0 : (DU * MT) / (VE * kThreads);
     static_assert(KC || (KB == NLD && KB >= 2), "bad transposing geometry");
     static_assert(KC || MODE != kF8, "fp8 operands must be K-contiguous");
+    // LDS write instructions per K-tile
+    static constexpr int NSTORE = KC ? (MODE == kX3 ? 2 * NLD : NLD)
+                                     : (ES == 2 ? 8 : (MODE == kX3 ? 8 : (KB == 2 ? 4 : 4 * (KB / 4))));
 
     __amdgpu_buffer_rsrc_t rsrc;
     uint32_t voff[NLD];     // byte offset of each load for K-tile 0 (kOOB if masked)
@@ -139,8 +142,8 @@ struct Loader {
                         uint32_t hw[KB / 2], lw[KB / 2];
 #pragma unroll
                         for (int p = 0; p < KB / 2; ++p) {
-                            float a = __builtin_bit_cast(float, v[2 * p][i]);
-                            float b = __builtin_bit_cast(float, v[2 * p + 1][i]);
+                            float a = as_float(v[2 * p][i]);
+                            float b = as_float(v[2 * p + 1][i]);
                             split2(a, b, hw[p], lw[p]);
                         }
                         char* dh = stage + lds_addr(lr, kg * KB * 2);
@@ -170,8 +173,8 @@ struct Loader {
     }
     __device__ static inline void split4(i32x4_t x, i32x2_t& hi, i32x2_t& lo) {
         uint32_t h0, l0, h1, l1;
-        split2(__builtin_bit_cast(float, x[0]), __builtin_bit_cast(float, x[1]), h0, l0);
-        split2(__builtin_bit_cast(float, x[2]), __builtin_bit_cast(float, x[3]), h1, l1);
+        split2(as_float(x[0]), as_float(x[1]), h0, l0);
+        split2(as_float(x[2]), as_float(x[3]), h1, l1);
         hi = i32x2_t{(int)h0, (int)h1};
         lo = i32x2_t{(int)l0, (int)l1};
     }
@@ -220,8 +223,7 @@ __device__ inline void mfma(const Frag<MODE>& a, const Frag<MODE>& b, f32x16_t& 
     } else if constexpr (MODE == kF32) {
 #pragma unroll
         for (int e = 0; e < 4; ++e)
-            acc = __builtin_amdgcn_mfma_f32_32x32x2f32(__builtin_bit_cast(float, a.x[e]),
-                                                       __builtin_bit_cast(float, b.x[e]), acc, 0, 0, 0);
+            acc = __builtin_amdgcn_mfma_f32_32x32x2f32(as_float(a.x[e]), as_float(b.x[e]), acc, 0, 0, 0);
     } else if constexpr (MODE == kX3) {
         // small cross terms first, then the dominant hi*hi product
         acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(__builtin_bit_cast(bf16x8_t, a.lo),
@@ -235,6 +237,27 @@ __device__ inline void mfma(const Frag<MODE>& a, const Frag<MODE>& b, f32x16_t& 
         i32x8_t bv = {b.x0[0], b.x0[1], b.x0[2], b.x0[3], b.x1[0], b.x1[1], b.x1[2], b.x1[3]};
         // format 0 = OCP e4m3 for both operands; e8m0 scale 127 = 1.0
         acc = __builtin_amdgcn_mfma_scale_f32_32x32x64_f8f6f4(av, bv, acc, 0, 0, 0, 127, 0, 127);
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Instruction interleaving for one read step: the MFMAs of the step with the
+// LDS fragment reads of the next step spread between them, plus `nv` global
+// loads and `nw` LDS writes (sched_group_barrier masks: 0x008 MFMA,
+// 0x020 VMEM read, 0x100 DS read, 0x200 DS write).
+// ---------------------------------------------------------------------------
+template <int MODE, int TM, int TN, int SYNC = 0>
+__device__ inline void sched_step(int nv, int nw) {
+    constexpr int MPS = (MODE == kF32) ? 4 : (MODE == kX3 ? 3 : 1);
+    constexpr int RPF = (MODE == kX3 || MODE == kF8) ? 2 : 1;
+    constexpr int NM = TM * TN * MPS;
+    constexpr int NR = (TM + TN) * RPF;
+#pragma unroll
+    for (int i = 0; i < NM; ++i) {
+        __builtin_amdgcn_sched_group_barrier(0x008, 1, SYNC);
+        if (i < nv) __builtin_amdgcn_sched_group_barrier(0x020, 1, SYNC);
+        if ((i * NR) / NM != ((i + 1) * NR) / NM) __builtin_amdgcn_sched_group_barrier(0x100, 1, SYNC);
+        if (i >= NM / 2 && (i - NM / 2) < nw) __builtin_amdgcn_sched_group_barrier(0x200, 1, SYNC);
     }
 }
 
@@ -318,10 +341,14 @@ __device__ inline void store16(const TileArgs& a, const char* Cb, char* Db, int 
     }
 }
 
-// Extent in bytes of one batch's operand as a buffer (clamped to 2 GiB - 16;
-// the host rejects problems that do not fit).
+// Extent in bytes of one batch's operand as a buffer: the last row ends at
+// `tail` elements, rounded up to the 16-byte vector that holds it (a partly
+// out-of-range vector would read as all zeros; an aligned 16-byte vector never
+// crosses a page, so the rounded extent stays inside mapped memory).  Clamped
+// to 2 GiB - 16; the host rejects problems that do not fit.
 __device__ inline uint32_t extent_bytes(int64_t rows, int64_t cols_ld, int64_t tail, int es) {
-    int64_t e = ((rows - 1) * cols_ld + tail) * es;
+    const int64_t ve = 16 / es;
+    int64_t e = ((rows - 1) * cols_ld + (tail + ve - 1) / ve * ve) * es;
     if (e < 0) e = 0;
     if (e > 0x7ffffff0ll) e = 0x7ffffff0ll;
     return (uint32_t)e;
@@ -367,46 +394,58 @@ __device__ inline void gemm_tile(const TileArgs& a, int tm, int tn, int b, char*
     const int arow = wm * WT_M + (lane & 31);
     const int brow = wn * WT_N + (lane & 31);
 
-    if (KT > 0) {
-        la.load(0);
-        lb.load(0);
-        la.store(smem);
-        lb.store(smem + MT_M * kRowBytes);
-        lds_barrier();
-    }
+    // Software pipeline (one LDS barrier per K-tile):
+    //   step 0      : loads of K-tile kt+1 -> VGPRs, interleaved with MFMAs
+    //   step rs     : LDS reads of step rs+1 interleaved with the MFMAs of step rs
+    //   step RS-2   : VGPRs of K-tile kt+1 -> the other LDS stage
+    //   barrier     : then the MFMAs of the last step run while the first
+    //                 fragments of K-tile kt+1 are read, so the MFMA pipe does
+    //                 not drain across the barrier.
+    // Loads past K are masked to zero, so the loop has no branches.
+    Frag<MODE> fa[2][TM], fb[2][TN];
+    la.load(0);
+    lb.load(0);
+    la.store(smem);
+    lb.store(smem + MT_M * kRowBytes);
+    lds_barrier();
+#pragma unroll
+    for (int i = 0; i < TM; ++i) read_frag<MODE>(smem, arow + 32 * i, g, 0, fa[0][i]);
+#pragma unroll
+    for (int j = 0; j < TN; ++j) read_frag<MODE>(smem + MT_M * kRowBytes, brow + 32 * j, g, 0, fb[0][j]);
 
     for (int kt = 0; kt < KT; ++kt) {
         const char* sa = smem + (kt & 1) * STAGE;
         const char* sb = sa + MT_M * kRowBytes;
-        const bool more = (kt + 1) < KT;
-        if (more) {
-            la.load(kt + 1);
-            lb.load(kt + 1);
-        }
-        Frag<MODE> fa[2][TM], fb[2][TN];
+        char* na = smem + ((kt + 1) & 1) * STAGE;
+        la.load(kt + 1);
+        lb.load(kt + 1);
 #pragma unroll
-        for (int i = 0; i < TM; ++i) read_frag<MODE>(sa, arow + 32 * i, g, 0, fa[0][i]);
+        for (int rs = 0; rs + 1 < RS; ++rs) {
 #pragma unroll
-        for (int j = 0; j < TN; ++j) read_frag<MODE>(sb, brow + 32 * j, g, 0, fb[0][j]);
+            for (int i = 0; i < TM; ++i) read_frag<MODE>(sa, arow + 32 * i, g, rs + 1, fa[(rs + 1) & 1][i]);
 #pragma unroll
-        for (int rs = 0; rs < RS; ++rs) {
-            if (rs + 1 < RS) {
-#pragma unroll
-                for (int i = 0; i < TM; ++i) read_frag<MODE>(sa, arow + 32 * i, g, rs + 1, fa[(rs + 1) & 1][i]);
-#pragma unroll
-                for (int j = 0; j < TN; ++j) read_frag<MODE>(sb, brow + 32 * j, g, rs + 1, fb[(rs + 1) & 1][j]);
-            }
+            for (int j = 0; j < TN; ++j) read_frag<MODE>(sb, brow + 32 * j, g, rs + 1, fb[(rs + 1) & 1][j]);
 #pragma unroll
             for (int i = 0; i < TM; ++i)
 #pragma unroll
                 for (int j = 0; j < TN; ++j) mfma<MODE>(fa[rs & 1][i], fb[rs & 1][j], acc[i][j]);
-        }
-        if (more) {
-            char* na = smem + ((kt + 1) & 1) * STAGE;
-            la.store(na);
-            lb.store(na + MT_M * kRowBytes);
+            if (rs + 2 == RS) {
+                la.store(na);
+                lb.store(na + MT_M * kRowBytes);
+            }
+            sched_step<MODE, TM, TN>(rs == 0 ? la.NLD + lb.NLD : 0,
+                                     rs + 2 == RS ? la.NSTORE + lb.NSTORE : 0);
         }
         lds_barrier();
+#pragma unroll
+        for (int i = 0; i < TM; ++i) read_frag<MODE>(na, arow + 32 * i, g, 0, fa[0][i]);
+#pragma unroll
+        for (int j = 0; j < TN; ++j) read_frag<MODE>(na + MT_M * kRowBytes, brow + 32 * j, g, 0, fb[0][j]);
+#pragma unroll
+        for (int i = 0; i < TM; ++i)
+#pragma unroll
+            for (int j = 0; j < TN; ++j) mfma<MODE>(fa[(RS - 1) & 1][i], fb[(RS - 1) & 1][j], acc[i][j]);
+        sched_step<MODE, TM, TN>(0, 0);
     }
 
     // epilogue
