@@ -2,8 +2,9 @@
 """Benchmark of the gemmMfma path on one MI355X.
 
 Headline workload (one "step" = one GEMM launch): bf16 GEMM with fp32
-accumulate, column-major NN, M = N = K = 4096, alpha = 1, beta = 0, A and B
-i.i.d. normal(0, 1/sqrt(K)) from seed 1, inputs resident in HBM.  The JSON
+accumulate in the reference's layout (TN: A and B K-contiguous, column-major
+D), M = N = K = 4096, alpha = 1, beta = 0, A and B i.i.d. normal(0, 1/sqrt(K))
+from seed 1, inputs resident in HBM.  NN and NT run under "extras".  The JSON
 line reports TFLOP/s (2*M*N*K per launch) and the fraction of the bf16 MFMA
 peak, the roofline of the dominant kernel measured with HIP events on the
 launch stream, the CPU reference GEMM timed on the host cores beside it, and
@@ -80,7 +81,7 @@ def main():
     A = (torch.randn(M * K, generator=g) / math.sqrt(K)).to(torch.bfloat16).to(dev)
     B = (torch.randn(K * N, generator=g) / math.sqrt(K)).to(torch.bfloat16).to(dev)
     D = torch.empty(M * N, dtype=torch.bfloat16, device=dev)
-    prob = tl.make_problem("NN", C.BF16, M, N, K, M, K, M, M)
+    prob = tl.make_problem("TN", C.BF16, M, N, K, K, K, M, M)
     tile = tl.query_tile(prob)
     fn = lambda: tl.run(prob, A, B, None, D)
 
@@ -113,13 +114,13 @@ def main():
         "vs_baseline": None,
         "dtype": "bf16",
         "data": "synthetic: A,B ~ normal(0,1/sqrt(K)) seed 1 (torch CPU generator), resident in HBM",
-        "config": {"workload": "gemm bf16 NN M=N=K=4096 alpha=1 beta=0, fp32 accumulate, bf16 out",
-                   "M": M, "N": N, "K": K, "layout": "NN", "tile": [tile.mt_m, tile.mt_n, tile.depth_u],
+        "config": {"workload": "gemm bf16 TN M=N=K=4096 alpha=1 beta=0, fp32 accumulate, bf16 out",
+                   "M": M, "N": N, "K": K, "layout": "TN", "tile": [tile.mt_m, tile.mt_n, tile.depth_u],
                    "parallelism": "single GPU"},
         "pct_of_peak": round(100.0 * tflops / PEAK_TFLOPS["bf16"], 2),
         "roofline": {"bound": "mfma", "achieved": round(tflops, 2), "peak": PEAK_TFLOPS["bf16"],
                      "unit": "TFLOP/s", "frac": round(tflops / PEAK_TFLOPS["bf16"], 4),
-                     "traffic": None, "kernel": "ffgemm::gemm_kernel<bf16,NN,%dx%d>" % (tile.mt_m, tile.mt_n),
+                     "traffic": None, "kernel": "ffgemm::v3::gemm_kernel<bf16,TN,%dx%d>" % (tile.mt_m, tile.mt_n),
                      "algorithmic_flop_per_launch": flop, "wall_s": round(wall, 4)},
     }
 
@@ -150,7 +151,7 @@ def run_extras(a, torch, np, tl, C, dev):
         B = (torch.randn(K * N, generator=g) / math.sqrt(K)).to(tdt).to(dev)
         C0 = (torch.rand(M * N, generator=g) * 2 - 1).to(tdt).to(dev)
         D = torch.empty(M * N, dtype=tdt, device=dev)
-        for layout in ("NN", "NT", "TN"):
+        for layout in ("NN", "NT", "TN", "TT"):
             for beta in (0.0, 1.0):
                 name = f"{dt}_{layout}_beta{int(beta)}_4096"
                 if not want(name):
@@ -210,6 +211,30 @@ def run_extras(a, torch, np, tl, C, dev):
         ex["precision_8192"] = res
         del A, B, D
 
+    # differentiation: bf16 C = A @ B, M=8192 N=4096 K=4096, forward NN and
+    # backward TN (dA = dC B^T) + NT (dB = A^T dC) through the autograd Function
+    if want("autograd_bf16"):
+        from tensilelite_amd.autograd import matmul
+        Mg, Ng, Kg = 8192, 4096, 4096
+        ga = torch.Generator(device="cpu").manual_seed(4)
+        a_ = torch.randn(Mg, Kg, generator=ga).to(torch.bfloat16).to(dev).requires_grad_()
+        b_ = torch.randn(Kg, Ng, generator=ga).to(torch.bfloat16).to(dev).requires_grad_()
+        dc = torch.randn(Mg, Ng, generator=ga).to(torch.bfloat16).to(dev)
+
+        def step():
+            c_ = matmul(a_, b_)
+            a_.grad = None
+            b_.grad = None
+            c_.backward(dc)
+
+        ms_fb = time_launches(step, max(5, steps // 4), 3, torch)
+        ms_f = time_launches(lambda: matmul(a_.detach(), b_.detach()), max(5, steps // 4), 3, torch)
+        fl = 2.0 * Mg * Ng * Kg
+        ex["autograd_bf16_8192x4096x4096"] = {"fwd_tflops": round(fl / (ms_f * 1e-3) / 1e12, 1),
+                                              "fwd_bwd_tflops": round(3 * fl / (ms_fb * 1e-3) / 1e12, 1),
+                                              "fwd_ms": round(ms_f, 4), "fwd_bwd_ms": round(ms_fb, 4)}
+        del a_, b_, dc
+
     # grouped: 64 problems, one persistent launch, hipGraph replay
     if want("grouped_64"):
         from tensilelite_amd import grouped
@@ -220,19 +245,20 @@ def run_extras(a, torch, np, tl, C, dev):
 def cpu_baseline(np, gpu_ms):
     """The oracle's restatement of the reference CPU GEMM (src/gemmMfma.cpp:
     325-346), fp32, k innermost, timed on this host: a bounded sample of the
-    headline workload (64 of the 4096 output columns of the 4096^3 NN GEMM)
-    on all cores, plus the reference's own CPU case (480x512x512) on 1 core."""
+    headline workload (1024 of the 4096 output columns of the 4096^3 TN GEMM,
+    about 10 s of CPU work) on all cores, plus the reference's own CPU case
+    (480x512x512) on 1 core."""
     sys.path.insert(0, os.path.dirname(os.path.abspath(__file__)))
     import oracle
 
     cores = len(os.sched_getaffinity(0))
     M = K = 4096
-    ncols = 64
+    ncols = 1024
     rng = np.random.default_rng(1)
     A = (rng.standard_normal(M * K) / 64.0).astype(np.float32)
     B = (rng.standard_normal(K * ncols) / 64.0).astype(np.float32)
     t0 = time.perf_counter()
-    oracle.gemm_f32("NN", M, ncols, K, 1.0, A, M, B, K, nthreads=cores)
+    oracle.gemm_f32("TN", M, ncols, K, 1.0, A, K, B, K, nthreads=cores)
     dt = time.perf_counter() - t0
     tf = 2.0 * M * ncols * K / dt / 1e12
     # reference CPU case: fp32 NN 480x512x512, U[-1,1] seed 0, one core, median of 3
@@ -247,7 +273,7 @@ def cpu_baseline(np, gpu_ms):
         ts.append(time.perf_counter() - t1)
     t1 = sorted(ts)[1]
     return {"value": round(tf, 6), "unit": "TFLOP/s", "cores": cores, "kind": "port",
-            "sample": f"4096x{ncols}x4096 fp32 NN (64 of 4096 columns of the headline GEMM), "
+            "sample": f"4096x{ncols}x4096 fp32 TN ({ncols} of 4096 columns of the headline GEMM), "
                       f"{dt:.2f} s on {cores} threads",
             "gpu_speedup": round((2.0 * 4096 ** 3 / (gpu_ms * 1e-3) / 1e12) / tf, 1),
             "config1_480x512x512_1core_s": round(t1, 4),

@@ -64,10 +64,12 @@ bool tile_supported(int m, int n) {
     return false;
 }
 
-// Resident workgroups per CU for a tile (VGPR/AGPR and LDS limited).
-int tile_occupancy(int m, int n) {
+// Resident workgroups per CU for a 16-bit tile (VGPR/AGPR and LDS limited):
+// 256 x 128 fits twice only when both operands are K-contiguous (TN).
+int tile_occupancy(int m, int n, int layout) {
     if (m == 64 && n == 64) return 4;
     if (m == 128 && n == 128) return 2;
+    if (m != n && layout == FFGEMM_TN) return 2;
     return 1;
 }
 
@@ -432,7 +434,7 @@ int ffgemm_grouped_plan_create(ffgemm_grouped_plan* plan, int count, const ffgem
     ffgemm_grouped_plan_s* P = new ffgemm_grouped_plan_s();
     P->count = (int)sorted.size();
     P->total = total;
-    P->grid = (int)std::max<int64_t>(1, std::min<int64_t>(total, (int64_t)cu_count() * tile_occupancy(t.m, t.n)));
+    P->grid = (int)std::max<int64_t>(1, std::min<int64_t>(total, (int64_t)cu_count() * tile_occupancy(t.m, t.n, layout)));
     P->mode = mode; P->out = out; P->layout = layout; P->tile = t;
     hipGetDevice(&P->device);
     if (P->count > 0) {

@@ -1,23 +1,22 @@
-// Deep-pipelined tile engine for 16-bit operands (bf16 / fp16).
+// Pipelined tile engine for 16-bit operands (bf16 / fp16).
 //
-// Why a second engine: with one workgroup of 4 waves per CU (one wave per
-// SIMD, 256 accumulator registers each), nothing hides global-load latency
-// except the software pipeline itself.  Measured on the first engine (one
-// K-tile of register prefetch): MFMA busy 43%, LDS active 30%, HBM far from
-// its roofline -- the waves were waiting on loads.  This engine keeps three
-// K-tiles in flight:
-//  * K = 32 per iteration, LDS ring of 4 stages of (MT_M + MT_N) rows x 64 B
-//    (128 KiB for 256 x 256), chunk index XOR-swizzled per row;
+// Measured on MI355X (4096^3, 256 x 256 tiles, one workgroup per CU): the
+// panel stream HBM/MALL -> L2 -> LDS is the bottleneck, and its throughput
+// depends on the request shape: a CU fetching whole 128-byte row segments
+// sustains 25 TB/s aggregate (LDS-direct), 64-byte segments only 14.7 TB/s.
+// Design:
+//  * LDS rows of RB bytes of K per stage (64 here; the 256 x 256 engine in
+//    gemm_tile_v3.hpp uses 128-byte rows), chunk index XOR-swizzled per row
+//    so 16-byte MFMA operand reads and 16-byte writes are conflict-free;
 //  * K-contiguous operands go straight from HBM to LDS
-//    (buffer_load_dwordx4 ... lds), issued three K-tiles ahead; each lane
-//    fetches the 16-byte chunk whose swizzled LDS slot is lane*16, so the
-//    swizzle costs nothing;
-//  * M/N-contiguous operands are loaded three K-tiles ahead into a 3-slot
-//    register ring and transposed into LDS (v_perm with per-lane selectors,
-//    row order rotated by strip parity so the stores are conflict-free) one
-//    K-tile ahead;
-//  * one barrier per K-tile; the MFMAs of the second half of K-tile kt-1 are
-//    issued after the barrier, while the fragments of K-tile kt load.
+//    (buffer_load_dwordx4 ... lds), PF K-tiles ahead; each lane fetches the
+//    16-byte chunk whose swizzled LDS slot is lane*16, so the swizzle is free;
+//  * M/N-contiguous operands are loaded PF K-tiles ahead into a register ring
+//    and transposed into LDS (v_perm with per-lane selectors; row order
+//    rotated by strip parity so the stores are conflict-free) one K-tile ahead;
+//  * one barrier per K-tile; every read step's fragments are loaded while the
+//    MFMAs of the previous step run, and the last step of K-tile kt-1 runs
+//    after the barrier, while the first fragments of K-tile kt load.
 // Loads beyond K, M or N read zero (buffer range checks), so the loop body
 // has no data-dependent branches.
 #pragma once
@@ -28,28 +27,30 @@
 namespace ffgemm {
 namespace v2 {
 
-constexpr int RB = 64;  // LDS bytes per row per stage
-constexpr int DU = 32;  // K per iteration (16-bit elements)
-
-// Pipeline depth and residency per macro tile.  NS LDS stages keep PF = NS-1
-// K-tiles in flight.  256 x 256 fills a CU alone (256 accumulators per lane,
-// 128 KiB LDS); the smaller tiles run two workgroups per CU so one group's
-// MFMAs cover the other's barrier and LDS waits.
-// A 256 x 128 tile with a transposing operand needs its register ring and
-// stays at one workgroup per CU (four stages); with two direct operands it
-// fits two per CU (three stages, 2 x 72 KiB).
+// Per-tile pipeline configuration: 64-byte rows (K = 32 per stage), 3-4
+// stages; two workgroups per CU where the registers allow, so one group's
+// MFMAs cover the other's waits.
 template <int MT_M, int MT_N, bool AK, bool BK>
 struct Cfg {
     static constexpr bool kBig = (MT_M == 256 && MT_N == 256);
     static constexpr bool kSmall = (MT_M == 128 && MT_N == 128);
+    // (256 x 256 with a K-contiguous B runs the split-ring engine instead;
+    // with an M/N-contiguous B, 64-byte rows and four stages measured best)
+    static constexpr int RB = 64;                      // LDS bytes per row per stage
+    static constexpr int DU = RB / 2;                  // K per stage (16-bit)
+    static constexpr int RS = DU / 16;                 // MFMA read steps per stage
     static constexpr int OCC = kBig ? 1 : ((kSmall || (AK && BK)) ? 2 : 1);
-    static constexpr int NS = (OCC == 2 && !kSmall) ? 3 : 4;
-    static constexpr int PF = NS - 1;
+    static constexpr int NS = kBig ? 4 : ((OCC == 2 && !kSmall) ? 3 : 4);
+    static constexpr int PF = NS - 1;                  // K-tiles in flight
 };
 
-__host__ __device__ constexpr int swz4(int r) { return ((r >> 1) ^ (r >> 3)) & 3; }
+template <int RB>
+__host__ __device__ constexpr int swz(int r) {
+    return RB == 128 ? ((r ^ (r >> 3)) & 7) : (((r >> 1) ^ (r >> 3)) & 3);
+}
+template <int RB>
 __host__ __device__ constexpr int lds_addr(int row, int off) {
-    return row * RB + ((((off >> 4) ^ swz4(row)) & 3) << 4) + (off & 15);
+    return row * RB + ((((off >> 4) ^ swz<RB>(row)) & (RB / 16 - 1)) << 4) + (off & 15);
 }
 // Inverse of perm_a_row: LDS row -> row of the A tile it holds.
 __host__ __device__ constexpr int perm_a_inv(int lr) {
@@ -60,12 +61,16 @@ __host__ __device__ constexpr int perm_a_inv(int lr) {
 
 typedef __attribute__((address_space(3))) void* lds_ptr_t;
 
-// K-contiguous operand: HBM -> LDS directly, 16 bytes per lane.
-template <int MT, bool PERM, int PF>
+// K-contiguous operand: HBM -> LDS directly, 16 bytes per lane.  One
+// instruction fills 1 KiB of LDS = 1024/RB rows.
+template <int MT, bool PERM, int PF, int RB>
 struct DirectLoader {
     static constexpr bool kDirect = true;
-    static constexpr int NI = MT / 64;  // buffer_load ... lds per wave per K-tile
+    static constexpr int DU = RB / 2;
+    static constexpr int LPR = RB / 16;              // lanes per row
+    static constexpr int NI = MT * RB / 4096;        // instructions per wave per K-tile
     static constexpr int NVMEM = NI;
+    static constexpr int NSTORE = 0;
     __amdgpu_buffer_rsrc_t rsrc;
     uint32_t voff[NI];
     int32_t klim[NI];  // K - (first K of this lane's chunk); masked rows: 0
@@ -78,8 +83,8 @@ struct DirectLoader {
         const int lane = threadIdx.x & 63;
 #pragma unroll
         for (int j = 0; j < NI; ++j) {
-            const int lr = (wave * NI + j) * 16 + (lane >> 2);
-            const int c = (lane & 3) ^ swz4(lr);
+            const int lr = (wave * NI + j) * (64 / LPR) + lane / LPR;
+            const int c = (lane % LPR) ^ swz<RB>(lr);
             const int r = row0 + (PERM ? perm_a_inv(lr) : lr);
             klim[j] = (r < rows) ? K - c * 8 : 0;
             voff[j] = (uint32_t)(((int64_t)r * ld + c * 8) * 2);
@@ -90,7 +95,7 @@ struct DirectLoader {
         char* dst = region + wave * (NI * 1024);
 #pragma unroll
         for (int j = 0; j < NI; ++j) {
-            const uint32_t off = (kt * DU < klim[j]) ? voff[j] + (uint32_t)kt * (DU * 2) : kOOB;
+            const uint32_t off = (kt * DU < klim[j]) ? voff[j] + (uint32_t)kt * RB : kOOB;
             __builtin_amdgcn_raw_ptr_buffer_load_lds(rsrc, (lds_ptr_t)(dst + j * 1024), 16, off, 0, 0, 0);
         }
     }
@@ -98,15 +103,17 @@ struct DirectLoader {
     __device__ inline void store(char*) const {}
 };
 
-// M/N-contiguous operand: 16-byte loads of 8 consecutive M (N) values for
-// KB consecutive K, held in a 3-slot ring, transposed on the way to LDS.
-template <int MT, bool PERM, int PF>
+// M/N-contiguous operand: 16-byte loads of 8 consecutive M (N) values for KB
+// consecutive K, held in a PF-slot ring, transposed on the way to LDS.
+template <int MT, bool PERM, int PF, int RB>
 struct TransLoader {
     static constexpr bool kDirect = false;
-    static constexpr int KB = MT / 64;      // K rows per thread per K-tile
-    static constexpr int NKG = DU / KB;     // K groups across consecutive lanes
+    static constexpr int DU = RB / 2;
+    static constexpr int KB = DU * MT / 2048;        // K rows per thread per K-tile
+    static constexpr int NKG = DU / KB;              // K groups across consecutive lanes
     static constexpr int NVMEM = KB;
-    static_assert(KB == 2 || KB == 4, "transposing loader needs MT of 128 or 256");
+    static constexpr int NSTORE = 8;
+    static_assert(KB == 2 || KB == 4 || KB == 8, "transposing loader geometry");
     __amdgpu_buffer_rsrc_t rsrc;
     uint32_t voff[KB];
     int32_t klim[KB];
@@ -140,8 +147,8 @@ struct TransLoader {
         for (int j = 0; j < KB; ++j)
             v[S][j] = buf_load16(rsrc, (kt * DU < klim[j]) ? voff[j] + (uint32_t)kt * kstep : kOOB);
     }
-    // Row mc*8 + (i ^ (mc&1)) of the tile receives element i ^ (mc&1) of
-    // each of the KB loaded vectors.
+    // Row mc*8 + (i ^ (mc&1)) of the tile receives element i ^ (mc&1) of each
+    // of the KB loaded vectors.
     template <int S>
     __device__ inline void store(char* region) const {
 #pragma unroll
@@ -153,12 +160,24 @@ struct TransLoader {
 #pragma unroll
             for (int p = 0; p < KB / 2; ++p)
                 w[p] = __builtin_amdgcn_perm((uint32_t)v[S][2 * p + 1][i >> 1], (uint32_t)v[S][2 * p][i >> 1], sel);
-            char* dst = region + lds_addr(lr, kg * KB * 2);
-            if constexpr (KB == 4) *(i32x2_t*)dst = i32x2_t{(int)w[0], (int)w[1]};
+            char* dst = region + lds_addr<RB>(lr, kg * KB * 2);
+            if constexpr (KB == 8) *(i32x4_t*)dst = i32x4_t{(int)w[0], (int)w[1], (int)w[2], (int)w[3]};
+            else if constexpr (KB == 4) *(i32x2_t*)dst = i32x2_t{(int)w[0], (int)w[1]};
             else *(int*)dst = (int)w[0];
         }
     }
 };
+
+// Compile-time loop: f(integral_constant<int, 0>) ... f(integral_constant<int, N-1>).
+template <int I, int N, class F>
+__device__ inline void static_for_impl(F&& f) {
+    if constexpr (I < N) {
+        f(std::integral_constant<int, I>{});
+        static_for_impl<I + 1, N>(f);
+    }
+}
+template <int N, class F>
+__device__ inline void static_for(F&& f) { static_for_impl<0, N>(f); }
 
 // s_waitcnt vmcnt(N) with lgkmcnt/expcnt left free (gfx9 encoding).
 template <int N>
@@ -167,8 +186,9 @@ __device__ inline void wait_vmcnt() {
     __builtin_amdgcn_s_waitcnt((N & 15) | (7 << 4) | (15 << 8) | ((N >> 4) << 14));
 }
 
-// Interleave for one block of NM MFMAs: first the NR LDS reads one after each
-// MFMA, then `nx` instructions of class `mask` (VMEM loads or LDS writes).
+// Interleave for one block of NM MFMAs: the NR LDS reads one after each of
+// the first MFMAs, then `nx` instructions of class MASK (VMEM loads 0x020 or
+// LDS writes 0x200) one after each of the following MFMAs.
 template <int NM, int NR, int SYNC, int MASK>
 __device__ inline void sched_block(int nx) {
 #pragma unroll
@@ -193,14 +213,17 @@ __device__ inline void mfma16(const i32x4_t& a, const i32x4_t& b, f32x16_t& acc)
 template <int MODE, bool AK, bool BK, int MT_M, int MT_N, int OUT>
 __device__ inline void gemm_tile(const TileArgs& a, int tm, int tn, int b, char* smem) {
     static_assert(MODE == kBF16 || MODE == kF16, "16-bit engine");
+    using C = Cfg<MT_M, MT_N, AK, BK>;
+    constexpr int RB = C::RB, DU = C::DU, RS = C::RS, NS = C::NS, PF = C::PF;
     constexpr int WT_M = MT_M / 2, WT_N = MT_N / 2;
     constexpr int TM = WT_M / 32, TN = WT_N / 32;
     constexpr int STAGE = (MT_M + MT_N) * RB;
     constexpr int BOFF = MT_M * RB;
-    constexpr int NS = Cfg<MT_M, MT_N, AK, BK>::NS, PF = Cfg<MT_M, MT_N, AK, BK>::PF;
-    using LA = std::conditional_t<AK, DirectLoader<MT_M, true, PF>, TransLoader<MT_M, true, PF>>;
-    using LB = std::conditional_t<BK, DirectLoader<MT_N, false, PF>, TransLoader<MT_N, false, PF>>;
-    constexpr int NV = LA::NVMEM + LB::NVMEM;  // VMEM instructions per K-tile
+    using LA = std::conditional_t<AK, DirectLoader<MT_M, true, PF, RB>, TransLoader<MT_M, true, PF, RB>>;
+    using LB = std::conditional_t<BK, DirectLoader<MT_N, false, PF, RB>, TransLoader<MT_N, false, PF, RB>>;
+    constexpr int NV = LA::NVMEM + LB::NVMEM;        // VMEM instructions per K-tile
+    constexpr int NW = LA::NSTORE + LB::NSTORE;      // transposed LDS stores per K-tile
+    static_assert(RS % 2 == 0, "read steps pair up in the fragment double buffer");
 
     const int lane = threadIdx.x & 63;
     const int wave = threadIdx.x >> 6;
@@ -235,14 +258,11 @@ __device__ inline void gemm_tile(const TileArgs& a, int tm, int tn, int b, char*
     auto stage = [&](int kt) { return smem + (kt % NS) * STAGE; };
 
     // prologue: K-tiles 0..PF-1 in flight, K-tile 0 transposed into stage 0
-    la.template issue<0>(0, stage(0));
-    lb.template issue<0>(0, stage(0) + BOFF);
-    la.template issue<1>(1, stage(1));
-    lb.template issue<1>(1, stage(1) + BOFF);
-    if constexpr (PF > 2) {
-        la.template issue<2 % PF>(2, stage(2));
-        lb.template issue<2 % PF>(2, stage(2) + BOFF);
-    }
+    static_for<PF>([&](auto j) {
+        constexpr int J = decltype(j)::value;
+        la.template issue<J>(J, stage(J));
+        lb.template issue<J>(J, stage(J) + BOFF);
+    });
     la.template store<0>(stage(0));
     lb.template store<0>(stage(0) + BOFF);
 
@@ -251,46 +271,48 @@ __device__ inline void gemm_tile(const TileArgs& a, int tm, int tn, int b, char*
         const char* sa = stage(kt);
         const char* sb = sa + BOFF;
         wait_vmcnt<(PF - 1) * NV>();  // K-tile kt's direct loads have landed (this wave)
-        lds_barrier();         // ... and every wave's; stage kt-1 is free again
+        lds_barrier();                // ... and every wave's; stage kt-1 is free again
         la.template issue<S>(kt + PF, stage(kt + PF));
         lb.template issue<S>(kt + PF, stage(kt + PF) + BOFF);
+        static_for<RS>([&](auto step) {
+            constexpr int R = decltype(step)::value;
+            constexpr int CUR = R & 1, PREV = (R + 1) & 1;
+            // fragments of read step R: lane group g takes chunk 2R + g
 #pragma unroll
-        for (int i = 0; i < TM; ++i) fa[0][i] = *(const i32x4_t*)(sa + lds_addr(arow + 32 * i, g * 16));
+            for (int i = 0; i < TM; ++i)
+                fa[CUR][i] = *(const i32x4_t*)(sa + lds_addr<RB>(arow + 32 * i, (2 * R + g) * 16));
 #pragma unroll
-        for (int j = 0; j < TN; ++j) fb[0][j] = *(const i32x4_t*)(sb + lds_addr(brow + 32 * j, g * 16));
-        // second half of K-tile kt-1 (zeros on the first iteration)
+            for (int j = 0; j < TN; ++j)
+                fb[CUR][j] = *(const i32x4_t*)(sb + lds_addr<RB>(brow + 32 * j, (2 * R + g) * 16));
+            // MFMAs of the previous step (for R = 0: the last step of K-tile
+            // kt-1, zeros on the first iteration)
 #pragma unroll
-        for (int i = 0; i < TM; ++i)
+            for (int i = 0; i < TM; ++i)
 #pragma unroll
-            for (int j = 0; j < TN; ++j) mfma16<MODE>(fa[1][i], fb[1][j], acc[i][j]);
-        sched_block<TM * TN, TM + TN, 1, 0x020>(NV);
-#pragma unroll
-        for (int i = 0; i < TM; ++i) fa[1][i] = *(const i32x4_t*)(sa + lds_addr(arow + 32 * i, (2 + g) * 16));
-#pragma unroll
-        for (int j = 0; j < TN; ++j) fb[1][j] = *(const i32x4_t*)(sb + lds_addr(brow + 32 * j, (2 + g) * 16));
-#pragma unroll
-        for (int i = 0; i < TM; ++i)
-#pragma unroll
-            for (int j = 0; j < TN; ++j) mfma16<MODE>(fa[0][i], fb[0][j], acc[i][j]);
-        // K-tile kt+1 (register path) into its stage
-        la.template store<(S + 1) % PF>(stage(kt + 1));
-        lb.template store<(S + 1) % PF>(stage(kt + 1) + BOFF);
-        sched_block<TM * TN, TM + TN, 2, 0x200>((LA::kDirect ? 0 : 8) + (LB::kDirect ? 0 : 8));
+                for (int j = 0; j < TN; ++j) mfma16<MODE>(fa[PREV][i], fb[PREV][j], acc[i][j]);
+            if constexpr (R == RS - 1) {
+                // K-tile kt+1 (register path) into its stage
+                la.template store<(S + 1) % PF>(stage(kt + 1));
+                lb.template store<(S + 1) % PF>(stage(kt + 1) + BOFF);
+                sched_block<TM * TN, TM + TN, R + 1, 0x200>(NW);
+            } else if constexpr (R == 0) {
+                sched_block<TM * TN, TM + TN, R + 1, 0x020>(NV);
+            } else {
+                sched_block<TM * TN, TM + TN, R + 1, 0x020>(0);
+            }
+        });
     };
 
     int kt = 0;
-    for (; kt + PF <= KT; kt += PF) {
-        iteration(std::integral_constant<int, 0>{}, kt);
-        iteration(std::integral_constant<int, 1>{}, kt + 1);
-        if constexpr (PF > 2) iteration(std::integral_constant<int, 2 % PF>{}, kt + 2);
-    }
-    if (kt < KT) iteration(std::integral_constant<int, 0>{}, kt++);
-    if constexpr (PF > 2)
-        if (kt < KT) iteration(std::integral_constant<int, 1>{}, kt++);
+    for (; kt + PF <= KT; kt += PF)
+        static_for<PF>([&](auto j) { iteration(j, kt + decltype(j)::value); });
+    static_for<PF - 1>([&](auto j) {
+        if (kt < KT) iteration(j, kt++);
+    });
 #pragma unroll
     for (int i = 0; i < TM; ++i)
 #pragma unroll
-        for (int j = 0; j < TN; ++j) mfma16<MODE>(fa[1][i], fb[1][j], acc[i][j]);
+        for (int j = 0; j < TN; ++j) mfma16<MODE>(fa[(RS - 1) & 1][i], fb[(RS - 1) & 1][j], acc[i][j]);
     // drain the loads issued for K-tiles past the end before the LDS is reused
     wait_vmcnt<0>();
     lds_barrier();
@@ -315,7 +337,10 @@ __device__ inline void gemm_tile(const TileArgs& a, int tm, int tn, int b, char*
 
 // LDS bytes the engine needs for a tile.
 template <int MT_M, int MT_N, bool AK, bool BK>
-__host__ __device__ constexpr int smem_bytes() { return Cfg<MT_M, MT_N, AK, BK>::NS * (MT_M + MT_N) * RB; }
+__host__ __device__ constexpr int smem_bytes() {
+    using C = Cfg<MT_M, MT_N, AK, BK>;
+    return C::NS * (MT_M + MT_N) * C::RB;
+}
 
 template <int MODE, bool AK, bool BK, int MT_M, int MT_N, int OUT>
 __global__ void __launch_bounds__(kThreads, (Cfg<MT_M, MT_N, AK, BK>::OCC)) gemm_kernel(TileArgs a) {

@@ -1,0 +1,219 @@
+// Tile engine for 256 x 256 tiles of 16-bit operands (bf16 / fp16).
+//
+// What limits this tile on MI355X is the panel stream into LDS (256 + 256
+// rows x K per tile, one tile per CU at 4096^3).  Two measured facts shape the
+// engine: 128-byte row segments per request stream about 1.7x faster than
+// 64-byte ones, and the stream only reaches that rate with roughly two K-tiles
+// (~96 KiB per CU) continuously in flight.  Three full 64 KiB stages do not
+// fit in 160 KiB of LDS, so the two operands are staged differently:
+//  * B: three LDS stages (96 KiB); K-contiguous B goes HBM -> LDS directly
+//    (buffer_load_dwordx4 ... lds) two K-tiles ahead; M/N-contiguous B goes
+//    through a 2-slot register ring and is transposed into LDS;
+//  * A: two LDS stages (64 KiB) fed from a 2-slot register ring loaded two
+//    K-tiles ahead (16-byte loads of whole 128-byte row segments, or the
+//    transposing loader when A is M-contiguous);
+//  * K = 64 per stage in 128-byte rows, XOR-swizzled 16-byte chunks;
+//  * one barrier per K-tile; the MFMAs of each read step run while the
+//    fragments of the next step load, and the last step of K-tile kt-1 runs
+//    after the barrier.
+#pragma once
+#include "gemm_tile_v2.hpp"
+
+namespace ffgemm {
+namespace v3 {
+
+using v2::DirectLoader;
+using v2::lds_addr;
+using v2::static_for;
+using v2::TransLoader;
+using v2::wait_vmcnt;
+
+constexpr int MT = 256;
+constexpr int RB = 128;          // bytes of K per LDS row per stage
+constexpr int DU = 64;           // K per stage
+constexpr int RS = 4;            // MFMA read steps per stage
+constexpr int NSA = 2, NSB = 3;  // LDS stages of A and B
+constexpr int STAGE = MT * RB;   // 32 KiB per operand stage
+constexpr int LDS_BYTES = (NSA + NSB) * STAGE;
+
+// K-contiguous operand through registers: thread t loads the 16-byte chunk
+// t%8 of rows t/8 + 32i (whole 128-byte row segments per 8 lanes).
+template <bool PERM, int PF>
+struct RegLoader {
+    static constexpr bool kDirect = false;
+    static constexpr int NLD = MT * RB / (16 * kThreads);  // 8
+    static constexpr int NVMEM = NLD;
+    static constexpr int NSTORE = NLD;
+    __amdgpu_buffer_rsrc_t rsrc;
+    uint32_t voff[NLD];
+    int32_t klim;
+    i32x4_t v[PF][NLD];
+
+    __device__ inline void init(const void* base, uint32_t extent, int32_t ld, int32_t rows, int32_t K,
+                                int32_t row0) {
+        rsrc = make_rsrc(base, extent);
+        const int t = threadIdx.x, c = t & 7;
+        klim = K - c * 8;
+#pragma unroll
+        for (int i = 0; i < NLD; ++i) {
+            const int r = row0 + (t >> 3) + 32 * i;
+            voff[i] = (r < rows) ? (uint32_t)(((int64_t)r * ld + c * 8) * 2) : kOOB;
+        }
+    }
+    template <int S>
+    __device__ inline void issue(int kt, char*) {
+        const bool ok = kt * DU < klim;
+#pragma unroll
+        for (int i = 0; i < NLD; ++i)
+            v[S][i] = buf_load16(rsrc, (ok && voff[i] != kOOB) ? voff[i] + (uint32_t)kt * RB : kOOB);
+    }
+    template <int S>
+    __device__ inline void store(char* region) const {
+        const int t = threadIdx.x, c = t & 7;
+#pragma unroll
+        for (int i = 0; i < NLD; ++i) {
+            const int r = (t >> 3) + 32 * i;
+            *(i32x4_t*)(region + lds_addr<RB>(PERM ? perm_a_row(r) : r, c * 16)) = v[S][i];
+        }
+    }
+};
+
+template <int NM, int NR, int SYNC, int MASK>
+__device__ inline void sched_block(int nx) { v2::sched_block<NM, NR, SYNC, MASK>(nx); }
+
+template <int MODE, bool AK, bool BK, int OUT>
+__device__ inline void gemm_tile(const TileArgs& a, int tm, int tn, int b, char* smem) {
+    static_assert(MODE == kBF16 || MODE == kF16, "16-bit engine");
+    constexpr int WT = MT / 2, T = WT / 32;  // 128 x 128 per wave, 4 x 4 MFMA tiles
+    // A: register ring of 2 slots (two K-tiles ahead) -> 2 LDS stages
+    using LA = std::conditional_t<AK, RegLoader<true, 2>, TransLoader<MT, true, 2, RB>>;
+    // B: direct into 3 LDS stages, or register ring -> 3 LDS stages
+    using LB = std::conditional_t<BK, DirectLoader<MT, false, 2, RB>, TransLoader<MT, false, 2, RB>>;
+    constexpr int NV = LA::NVMEM + LB::NVMEM;
+    constexpr int NW = LA::NSTORE + LB::NSTORE;
+
+    const int lane = threadIdx.x & 63;
+    const int wave = threadIdx.x >> 6;
+    const int wm = wave & 1, wn = wave >> 1;
+    const int m0 = tm * MT, n0 = tn * MT;
+    const int g = lane >> 5;
+
+    const char* Ab = (const char*)a.A + (int64_t)b * a.bsA * 2;
+    const char* Bb = (const char*)a.B + (int64_t)b * a.bsB * 2;
+    LA la;
+    LB lb;
+    la.init(Ab, AK ? extent_bytes(a.M, a.lda, a.K, 2) : extent_bytes(a.K, a.lda, a.M, 2), a.lda, a.M, a.K, m0);
+    lb.init(Bb, BK ? extent_bytes(a.N, a.ldb, a.K, 2) : extent_bytes(a.K, a.ldb, a.N, 2), a.ldb, a.N, a.K, n0);
+
+    f32x16_t acc[T][T];
+#pragma unroll
+    for (int i = 0; i < T; ++i)
+#pragma unroll
+        for (int j = 0; j < T; ++j)
+#pragma unroll
+            for (int e = 0; e < 16; ++e) acc[i][j][e] = 0.0f;
+    i32x4_t fa[2][T], fb[2][T];
+#pragma unroll
+    for (int i = 0; i < T; ++i) {
+        fa[1][i] = i32x4_t{0, 0, 0, 0};
+        fb[1][i] = i32x4_t{0, 0, 0, 0};
+    }
+
+    const int arow = wm * WT + (lane & 31);
+    const int brow = wn * WT + (lane & 31);
+    const int KT = (a.K + DU - 1) / DU;
+
+    char* const sA = smem;                  // NSA stages of A
+    char* const sB = smem + NSA * STAGE;    // NSB stages of B
+    auto stA = [&](int kt) { return sA + (kt & 1) * STAGE; };
+    auto stB = [&](int kt) { return sB + (kt % NSB) * STAGE; };
+
+    // prologue: K-tiles 0 and 1 in flight; K-tile 0 written to its stages
+    la.template issue<0>(0, nullptr);
+    lb.template issue<0>(0, stB(0));
+    la.template issue<1>(1, nullptr);
+    lb.template issue<1>(1, stB(1));
+    la.template store<0>(stA(0));
+    lb.template store<0>(stB(0));
+
+    auto iteration = [&](auto slot, int kt) {
+        constexpr int S = decltype(slot)::value;  // kt % 2
+        const char* sa = stA(kt);
+        const char* sb = stB(kt);
+        // B's direct loads of K-tile kt were issued two iterations ago; only
+        // the NV loads of the last iteration may still be in flight
+        if constexpr (BK) wait_vmcnt<NV>();
+        lds_barrier();  // K-tile kt complete in LDS; K-tile kt-1 fully consumed
+        // K-tile kt+2: A into register slot S (its K-tile kt is in LDS now),
+        // B into stage (kt+2)%3 (K-tile kt-1 is done)
+        la.template issue<S>(kt + 2, nullptr);
+        lb.template issue<S>(kt + 2, stB(kt + 2));
+        static_for<RS>([&](auto step) {
+            constexpr int R = decltype(step)::value;
+            constexpr int CUR = R & 1, PREV = (R + 1) & 1;
+#pragma unroll
+            for (int i = 0; i < T; ++i)
+                fa[CUR][i] = *(const i32x4_t*)(sa + lds_addr<RB>(arow + 32 * i, (2 * R + g) * 16));
+#pragma unroll
+            for (int j = 0; j < T; ++j)
+                fb[CUR][j] = *(const i32x4_t*)(sb + lds_addr<RB>(brow + 32 * j, (2 * R + g) * 16));
+#pragma unroll
+            for (int i = 0; i < T; ++i)
+#pragma unroll
+                for (int j = 0; j < T; ++j) v2::mfma16<MODE>(fa[PREV][i], fb[PREV][j], acc[i][j]);
+            if constexpr (R == RS - 1) {
+                // K-tile kt+1 from registers into its stages (A always; B when
+                // it is transposed)
+                la.template store<S ^ 1>(stA(kt + 1));
+                lb.template store<S ^ 1>(stB(kt + 1));
+                sched_block<T * T, 2 * T, R + 1, 0x200>(NW);
+            } else if constexpr (R == 0) {
+                sched_block<T * T, 2 * T, R + 1, 0x020>(NV);
+            } else {
+                sched_block<T * T, 2 * T, R + 1, 0x020>(0);
+            }
+        });
+    };
+
+    int kt = 0;
+    for (; kt + 2 <= KT; kt += 2) {
+        iteration(std::integral_constant<int, 0>{}, kt);
+        iteration(std::integral_constant<int, 1>{}, kt + 1);
+    }
+    if (kt < KT) iteration(std::integral_constant<int, 0>{}, kt);
+#pragma unroll
+    for (int i = 0; i < T; ++i)
+#pragma unroll
+        for (int j = 0; j < T; ++j) v2::mfma16<MODE>(fa[1][i], fb[1][j], acc[i][j]);
+    // drain the loads issued past the end before the LDS is reused
+    wait_vmcnt<0>();
+    lds_barrier();
+
+    const char* Cb = (const char*)a.C + (int64_t)b * a.bsC * ((OUT == kOutF32) ? 4 : 2);
+    char* Db = (char*)a.D + (int64_t)b * a.bsD * ((OUT == kOutF32) ? 4 : 2);
+#pragma unroll
+    for (int i = 0; i < T; ++i) {
+        const int m = m0 + wm * WT + 32 * i + 16 * g;
+#pragma unroll
+        for (int j = 0; j < T; ++j) {
+            const int n = n0 + wn * WT + 32 * j + (lane & 31);
+            if (n < a.N && m < a.M) {
+                float v[16];
+#pragma unroll
+                for (int e = 0; e < 16; ++e) v[e] = acc[i][j][e];
+                store16<OUT>(a, Cb, Db, m, n, v, a.alpha, a.beta);
+            }
+        }
+    }
+}
+
+template <int MODE, bool AK, bool BK, int OUT>
+__global__ void __launch_bounds__(kThreads, 1) gemm_kernel(TileArgs a) {
+    __shared__ __attribute__((aligned(16))) char smem[LDS_BYTES];
+    int tb, tm, tn;
+    map_tile(blockIdx.x, gridDim.x, a, tb, tm, tn);
+    v3::gemm_tile<MODE, AK, BK, OUT>(a, tm, tn, tb, smem);
+}
+
+}  // namespace v3
+}  // namespace ffgemm

@@ -13,10 +13,20 @@ namespace ffgemm {
 // 256 x 256 is left out: with the descriptor walk it exceeds the register file.
 #define FFGEMM_GROUPED_TILES(X) X(256, 128) X(128, 256) X(128, 128) X(64, 64)
 
+// bf16/fp16 tiles of 128 and up use the pipelined engine (v2::), the rest
+// the first engine.
+template <int MODE, bool AK, bool BK, int MT_M, int MT_N>
+struct GroupedEngine {
+    static constexpr bool kV2 = (MODE == kBF16 || MODE == kF16) && MT_M >= 128 && MT_N >= 128;
+    static constexpr int OCC = kV2 ? v2::Cfg<MT_M, MT_N, AK, BK>::OCC : 1;
+    static constexpr int LDS = kV2 ? v2::smem_bytes<MT_M, MT_N, AK, BK>() : 2 * (MT_M + MT_N) * kRowBytes;
+};
+
 template <int MODE, bool AK, bool BK, int MT_M, int MT_N, int OUT>
-__global__ void __launch_bounds__(kThreads, 1)
+__global__ void __launch_bounds__(kThreads, (GroupedEngine<MODE, AK, BK, MT_M, MT_N>::OCC))
     grouped_kernel(const GroupDesc* __restrict__ descs, int count, int64_t total_tiles) {
-    __shared__ __attribute__((aligned(16))) char smem[2 * (MT_M + MT_N) * kRowBytes];
+    using E = GroupedEngine<MODE, AK, BK, MT_M, MT_N>;
+    __shared__ __attribute__((aligned(16))) char smem[E::LDS];
     for (int64_t t = blockIdx.x; t < total_tiles; t += gridDim.x) {
         int lo = 0, hi = count - 1;
         while (lo < hi) {  // last descriptor with tile_start <= t
@@ -30,7 +40,8 @@ __global__ void __launch_bounds__(kThreads, 1)
         const int grp = local / gsize, first = grp * gm;
         const int h = min(d.a.tiles_m - first, gm);
         const int r = local - grp * gsize;
-        gemm_tile<MODE, AK, BK, MT_M, MT_N, OUT>(d.a, first + r % h, r / h, 0, smem);
+        if constexpr (E::kV2) v2::gemm_tile<MODE, AK, BK, MT_M, MT_N, OUT>(d.a, first + r % h, r / h, 0, smem);
+        else gemm_tile<MODE, AK, BK, MT_M, MT_N, OUT>(d.a, first + r % h, r / h, 0, smem);
     }
 }
 

@@ -98,10 +98,12 @@ void to_device_values(std::vector<float>& x, std::vector<uint16_t>& bits) {
     }
 }
 
+// fp32 data when `bits` is empty, else the 16-bit storage in `bits`
 void* upload(const std::vector<float>& x, const std::vector<uint16_t>& bits) {
-    size_t bytes = g.dtype == 1 ? x.size() * 4 : x.size() * 2;
+    const bool f32 = bits.empty();
+    const size_t bytes = f32 ? x.size() * 4 : bits.size() * 2;
     void* d = RuntimeHip::GetInstance()->DevMalloc(bytes + 256);
-    RuntimeHip::GetInstance()->MemCopyH2D(d, g.dtype == 1 ? (const void*)x.data() : (const void*)bits.data(), bytes);
+    if (d) RuntimeHip::GetInstance()->MemCopyH2D(d, f32 ? (const void*)x.data() : (const void*)bits.data(), bytes);
     return d;
 }
 
