@@ -29,8 +29,8 @@ HBM_GBPS = 8000.0
 def _args():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=200)
-    ap.add_argument("--warmup", type=int, default=20)
+    ap.add_argument("--steps", type=int, default=1000)
+    ap.add_argument("--warmup", type=int, default=200)
     ap.add_argument("--no-extras", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--only", default="", help="run only this extra (profiling)")
@@ -120,7 +120,8 @@ def main():
         "pct_of_peak": round(100.0 * tflops / PEAK_TFLOPS["bf16"], 2),
         "roofline": {"bound": "mfma", "achieved": round(tflops, 2), "peak": PEAK_TFLOPS["bf16"],
                      "unit": "TFLOP/s", "frac": round(tflops / PEAK_TFLOPS["bf16"], 4),
-                     "traffic": None, "kernel": "ffgemm::v3::gemm_kernel<bf16,TN,%dx%d>" % (tile.mt_m, tile.mt_n),
+                     "traffic": profiled_traffic(), "traffic_source": TRAFFIC_PROFILE,
+                     "kernel": "ffgemm::v3::gemm_kernel<bf16,TN,%dx%d>" % (tile.mt_m, tile.mt_n),
                      "algorithmic_flop_per_launch": flop, "wall_s": round(wall, 4)},
     }
 
@@ -132,6 +133,24 @@ def main():
     if not a.no_cpu:
         out["cpu_baseline"] = cpu_baseline(np, ms)
     print(json.dumps(out))
+
+
+TRAFFIC_PROFILE = "profiles/gemm_bf16_tn_4096_pmc.md"
+
+
+def profiled_traffic():
+    """HBM-side bytes per launch of the headline kernel, from the committed
+    rocprofv3 FETCH_SIZE / WRITE_SIZE passes (each scaled by a plain copy's
+    reading of the same counter); None when the summary is absent."""
+    path = os.path.join(os.path.dirname(os.path.abspath(__file__)), TRAFFIC_PROFILE)
+    try:
+        with open(path) as f:
+            for line in f:
+                if line.startswith("traffic_bytes_per_launch:"):
+                    return int(line.split(":", 1)[1])
+    except OSError:
+        pass
+    return None
 
 
 def run_extras(a, torch, np, tl, C, dev):
