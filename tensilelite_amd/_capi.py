@@ -9,7 +9,7 @@ import ctypes
 import os
 
 HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(HERE, "lib", "libffgemm.so")
+LIB_PATH = os.environ.get("FFGEMM_LIB") or os.path.join(HERE, "lib", "libffgemm.so")
 
 SUCCESS, WARN, ERR, FATAL = 0, 1, 2, 3
 NN, NT, TN, TT = 0, 1, 2, 3

@@ -1,0 +1,210 @@
+// Eight-wave ping-pong engine: 256 x 256 tiles of 16-bit operands (bf16 /
+// fp16) with both operands K-contiguous (TN, the reference's layout).
+//
+// Eight waves, two per SIMD.  The four waves of group 0 (waves 0-3) own
+// output rows 0-127 of the tile, group 1 (waves 4-7) rows 128-255; wave s of
+// a group owns 64 columns, a 128 x 64 block of 4 x 2 MFMA 32x32x16 tiles.
+// The groups run one "window" apart: while the wave of one group runs the 32
+// MFMAs of a K-tile (its compute window), the other wave on the same SIMD
+// loads its fragments for the next K-tile from LDS and issues the HBM -> LDS
+// loads (its memory window), so the MFMA pipe of every SIMD always has a wave
+// with ready operands.  One workgroup barrier closes every window.
+//
+// LDS (160 KiB): K = 64 per stage in 128-byte rows with XOR-swizzled 16-byte
+// chunks (conflict-free fragment reads), B in three stages and A in two.
+// Every load goes HBM -> LDS directly (buffer_load_dwordx4 ... lds) two
+// K-tiles ahead:
+//   * group g loads half g (128 rows) of B for K-tile kt+2 at the start of
+//     its memory window of K-tile kt (B stage (kt+2) % 3 was last read in
+//     K-tile kt-1), and
+//   * its own half g of A for K-tile kt+2 at the start of its compute window
+//     of K-tile kt (only group g reads that half, and it has just finished
+//     reading K-tile kt from it).
+// So every load has two to four windows to land before it is read.
+#pragma once
+#include "gemm_tile_v2.hpp"
+
+namespace ffgemm {
+namespace v4 {
+
+using v2::lds_addr;
+using v2::lds_ptr_t;
+using v2::wait_vmcnt;
+
+constexpr int kThreads8 = 512;
+constexpr int MT = 256;
+constexpr int RB = 128;  // bytes of K per LDS row per stage
+constexpr int DU = 64;   // K per stage
+constexpr int KS = 4;    // MFMA K-steps (16) per stage
+constexpr int STAGE = MT * RB;
+constexpr int NSA = 2, NSB = 3;
+constexpr int LDS_BYTES = (NSA + NSB) * STAGE;
+constexpr int TI = 4, TJ = 2;  // MFMA tiles per wave along M and N
+constexpr int NLD = 4;         // load instructions per wave per operand per K-tile
+
+// Half (128 rows) of a 256-row K-contiguous operand panel for one K-tile,
+// loaded by the four waves of one group: wave s fills LDS rows
+// 128h + 32s + 8q + lane/8 (q = 0..3), one 1 KiB instruction per q.  Lane
+// p of a row takes the logical chunk whose swizzled slot is p.  PERM applies
+// the A-row permutation (LDS row -> tile row perm_a_inv), which is 4 rows
+// per q inside a 32-row block instead of 8.
+template <bool PERM>
+struct HalfLoader {
+    static constexpr int QROWS = PERM ? 4 : 8;  // tile rows between q and q+1
+    __amdgpu_buffer_rsrc_t rsrc;
+    uint32_t base;    // byte offset of this lane's q = 0 row
+    int32_t rlim;     // rows - (this lane's q = 0 row)
+    int32_t x;        // logical chunk of q = 0 (chunk of q is x ^ q)
+    uint32_t qstride; // bytes between the rows of q and q + 1
+    int32_t K;
+    int32_t lds_off;  // this wave's first LDS row, in bytes
+
+    __device__ inline void init(const void* ptr, uint32_t extent, int32_t ld, int32_t rows, int32_t K_,
+                                int32_t row0, int h, int s) {
+        rsrc = make_rsrc(ptr, extent);
+        const int lane = threadIdx.x & 63;
+        const int j = lane >> 3;
+        const int lr0 = 128 * h + 32 * s + j;  // LDS row of q = 0
+        const int r = row0 + (PERM ? v2::perm_a_inv(lr0) : lr0);
+        rlim = rows - r;
+        x = ((lane & 7) ^ j ^ (4 * (s & 1))) & 7;
+        base = (uint32_t)((int64_t)r * ld * 2);
+        qstride = (uint32_t)(QROWS * ld * 2);
+        K = K_;
+        lds_off = (128 * h + 32 * s) * RB;
+    }
+    __device__ inline void issue(int kt, char* stage) const {
+        char* dst = stage + lds_off;
+#pragma unroll
+        for (int q = 0; q < NLD; ++q) {
+            const int c = x ^ q;
+            const bool ok = (QROWS * q < rlim) && (kt * DU + c * 8 < K);
+            const uint32_t off = ok ? base + q * qstride + (uint32_t)kt * RB + (uint32_t)c * 16 : kOOB;
+            __builtin_amdgcn_raw_ptr_buffer_load_lds(rsrc, (lds_ptr_t)(dst + q * 1024), 16, off, 0, 0, 0);
+        }
+    }
+};
+
+__device__ inline void barrier() { asm volatile("s_barrier" ::: "memory"); }
+
+template <int MODE, int OUT>
+__device__ inline void gemm_tile(const TileArgs& a, int tm, int tn, int b, char* smem) {
+    static_assert(MODE == kBF16 || MODE == kF16, "16-bit engine");
+    const int lane = threadIdx.x & 63;
+    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const int grp = wave >> 2, s = wave & 3;
+    const int h = lane >> 5;
+    const int m0 = tm * MT, n0 = tn * MT;
+
+    const char* Ab = (const char*)a.A + (int64_t)b * a.bsA * 2;
+    const char* Bb = (const char*)a.B + (int64_t)b * a.bsB * 2;
+    HalfLoader<true> la;
+    HalfLoader<false> lb;
+    la.init(Ab, extent_bytes(a.M, a.lda, a.K, 2), a.lda, a.M, a.K, m0, grp, s);
+    lb.init(Bb, extent_bytes(a.N, a.ldb, a.K, 2), a.ldb, a.N, a.K, n0, grp, s);
+
+    char* const sA = smem;
+    char* const sB = smem + NSA * STAGE;
+    auto stA = [&](int kt) { return sA + (kt & 1) * STAGE; };
+    auto stB = [&](int kt) { return sB + (kt % NSB) * STAGE; };
+
+    // Fragment addresses.  A row 128*grp + 32i + lane%32 and B row
+    // 64s + 32j + lane%32 both have swizzle (lane&7) ^ ((lane>>3)&3) ^ 4*(i|j & 1),
+    // so chunk 2ks + h sits in slot y ^ c with c = 2ks ^ 4*(i|j & 1): four
+    // per-lane offsets serve every fragment; the rest is uniform.
+    const int y = (h ^ (lane & 7) ^ ((lane >> 3) & 3)) & 7;
+    int P[4];
+#pragma unroll
+    for (int c = 0; c < 4; ++c) P[c] = (lane & 31) * RB + ((y ^ (2 * c)) << 4);
+    const int aoff = 128 * grp * RB, boff = 64 * s * RB;
+
+    f32x16_t acc[TI][TJ];
+#pragma unroll
+    for (int i = 0; i < TI; ++i)
+#pragma unroll
+        for (int j = 0; j < TJ; ++j)
+#pragma unroll
+            for (int e = 0; e < 16; ++e) acc[i][j][e] = 0.0f;
+    i32x4_t fa[2][TI], fb[2][TJ];
+
+    const int KT = (a.K + DU - 1) / DU;
+
+    // prologue: K-tiles 0 and 1 in flight; K-tile 0 complete before the barrier
+    lb.issue(0, stB(0));
+    la.issue(0, stA(0));
+    lb.issue(1, stB(1));
+    la.issue(1, stA(1));
+    wait_vmcnt<2 * NLD>();
+    lds_barrier();
+    if (grp == 1) barrier();  // group 1 runs one window behind group 0
+
+    // Windows per K-tile: memory(half 0), compute(half 0), memory(half 1),
+    // compute(half 1); a half is K = 32 (two MFMA K-steps).
+    for (int kt = 0; kt < KT; ++kt) {
+        const char* sa = stA(kt) + aoff;
+        const char* sb = stB(kt) + boff;
+#pragma unroll
+        for (int half = 0; half < 2; ++half) {
+            // ---- memory window
+            if (half == 0) lb.issue(kt + 2, stB(kt + 2));
+#pragma unroll
+            for (int k2 = 0; k2 < 2; ++k2) {
+                const int ks = 2 * half + k2;
+#pragma unroll
+                for (int i = 0; i < TI; ++i)
+                    fa[k2][i] = *(const i32x4_t*)(sa + P[ks ^ (2 * (i & 1))] + 4096 * i);
+#pragma unroll
+                for (int j = 0; j < TJ; ++j)
+                    fb[k2][j] = *(const i32x4_t*)(sb + P[ks ^ (2 * j)] + 4096 * j);
+            }
+            if (half == 1) wait_vmcnt<2 * NLD>();
+            lds_barrier();
+            __builtin_amdgcn_sched_barrier(0);
+            // ---- compute window
+            if (half == 1) la.issue(kt + 2, stA(kt));  // this group's half of A is consumed
+            __builtin_amdgcn_s_setprio(1);
+#pragma unroll
+            for (int k2 = 0; k2 < 2; ++k2)
+#pragma unroll
+                for (int i = 0; i < TI; ++i)
+#pragma unroll
+                    for (int j = 0; j < TJ; ++j) v2::mfma16<MODE>(fa[k2][i], fb[k2][j], acc[i][j]);
+            __builtin_amdgcn_s_setprio(0);
+            __builtin_amdgcn_sched_barrier(0);
+            if (half == 1) wait_vmcnt<2 * NLD>();
+            barrier();
+        }
+    }
+    // loads issued past the end write zeros into free stages; drain them
+    // before the workgroup's LDS is released
+    wait_vmcnt<0>();
+
+    const char* Cb = (const char*)a.C + (int64_t)b * a.bsC * ((OUT == kOutF32) ? 4 : 2);
+    char* Db = (char*)a.D + (int64_t)b * a.bsD * ((OUT == kOutF32) ? 4 : 2);
+#pragma unroll
+    for (int i = 0; i < TI; ++i) {
+        const int m = m0 + 128 * grp + 32 * i + 16 * h;
+#pragma unroll
+        for (int j = 0; j < TJ; ++j) {
+            const int n = n0 + 64 * s + 32 * j + (lane & 31);
+            if (n < a.N && m < a.M) {
+                float v[16];
+#pragma unroll
+                for (int e = 0; e < 16; ++e) v[e] = acc[i][j][e];
+                store16<OUT>(a, Cb, Db, m, n, v, a.alpha, a.beta);
+            }
+        }
+    }
+    if (grp == 0) barrier();  // matches group 1's extra barrier
+}
+
+template <int MODE, int OUT>
+__global__ void __launch_bounds__(kThreads8, 1) gemm_kernel(TileArgs a) {
+    __shared__ __attribute__((aligned(16))) char smem[LDS_BYTES];
+    int tb, tm, tn;
+    map_tile(blockIdx.x, gridDim.x, a, tb, tm, tn);
+    v4::gemm_tile<MODE, OUT>(a, tm, tn, tb, smem);
+}
+
+}  // namespace v4
+}  // namespace ffgemm
