@@ -1,5 +1,5 @@
-// Eight-wave ping-pong engine: 256 x 256 tiles of 16-bit operands (bf16 /
-// fp16) with both operands K-contiguous (TN, the reference's layout).
+// Eight-wave ping-pong engine: 256 x 256 tiles of bf16 / fp16 / fp8 (OCP
+// e4m3) operands with both operands K-contiguous (TN, the reference's layout).
 //
 // Eight waves, two per SIMD.  The four waves of group 0 (waves 0-3) own
 // output rows 0-127 of the tile, group 1 (waves 4-7) rows 128-255; wave s of
@@ -8,9 +8,11 @@
 // MFMAs of a K-tile (its compute window), the other wave on the same SIMD
 // loads its fragments for the next K-tile from LDS and issues the HBM -> LDS
 // loads (its memory window), so the MFMA pipe of every SIMD always has a wave
-// with ready operands.  One workgroup barrier closes every window.
+// with ready operands.  One workgroup barrier closes every window.  A window
+// covers 64 bytes of K: two v_mfma_f32_32x32x16 K-steps per tile (16-bit) or
+// one v_mfma_scale_f32_32x32x64_f8f6f4 (fp8), 512 MFMA cycles either way.
 //
-// LDS (160 KiB): K = 64 per stage in 128-byte rows with XOR-swizzled 16-byte
+// LDS (160 KiB): 128 bytes of K per stage and row, with XOR-swizzled 16-byte
 // chunks (conflict-free fragment reads), B in three stages and A in two.
 // Every load goes HBM -> LDS directly (buffer_load_dwordx4 ... lds) two
 // K-tiles ahead:
@@ -34,8 +36,6 @@ using v2::wait_vmcnt;
 constexpr int kThreads8 = 512;
 constexpr int MT = 256;
 constexpr int RB = 128;  // bytes of K per LDS row per stage
-constexpr int DU = 64;   // K per stage
-constexpr int KS = 4;    // MFMA K-steps (16) per stage
 constexpr int STAGE = MT * RB;
 constexpr int NSA = 2, NSB = 3;
 constexpr int LDS_BYTES = (NSA + NSB) * STAGE;
@@ -48,9 +48,11 @@ constexpr int NLD = 4;         // load instructions per wave per operand per K-t
 // p of a row takes the logical chunk whose swizzled slot is p.  PERM applies
 // the A-row permutation (LDS row -> tile row perm_a_inv), which is 4 rows
 // per q inside a 32-row block instead of 8.
-template <bool PERM>
+template <bool PERM, int ES = 2>
 struct HalfLoader {
     static constexpr int QROWS = PERM ? 4 : 8;  // tile rows between q and q+1
+    static constexpr int DU = RB / ES;          // K per stage
+    static constexpr int CE = 16 / ES;          // K per 16-byte chunk
     __amdgpu_buffer_rsrc_t rsrc;
     uint32_t base;    // byte offset of this lane's q = 0 row
     int32_t rlim;     // rows - (this lane's q = 0 row)
@@ -68,8 +70,8 @@ struct HalfLoader {
         const int r = row0 + (PERM ? v2::perm_a_inv(lr0) : lr0);
         rlim = rows - r;
         x = ((lane & 7) ^ j ^ (4 * (s & 1))) & 7;
-        base = (uint32_t)((int64_t)r * ld * 2);
-        qstride = (uint32_t)(QROWS * ld * 2);
+        base = (uint32_t)((int64_t)r * ld * ES);
+        qstride = (uint32_t)(QROWS * ld * ES);
         K = K_;
         lds_off = (128 * h + 32 * s) * RB;
     }
@@ -78,7 +80,7 @@ struct HalfLoader {
 #pragma unroll
         for (int q = 0; q < NLD; ++q) {
             const int c = x ^ q;
-            const bool ok = (QROWS * q < rlim) && (kt * DU + c * 8 < K);
+            const bool ok = (QROWS * q < rlim) && (kt * DU + c * CE < K);
             const uint32_t off = ok ? base + q * qstride + (uint32_t)kt * RB + (uint32_t)c * 16 : kOOB;
             __builtin_amdgcn_raw_ptr_buffer_load_lds(rsrc, (lds_ptr_t)(dst + q * 1024), 16, off, 0, 0, 0);
         }
@@ -87,21 +89,31 @@ struct HalfLoader {
 
 __device__ inline void barrier() { asm volatile("s_barrier" ::: "memory"); }
 
+// OCP e4m3 x e4m3, 32x32x64, unit block scales (e8m0 127 = 1.0).
+__device__ inline void mfma_f8(const i32x4_t& a0, const i32x4_t& a1, const i32x4_t& b0, const i32x4_t& b1,
+                               f32x16_t& acc) {
+    const i32x8_t av = {a0[0], a0[1], a0[2], a0[3], a1[0], a1[1], a1[2], a1[3]};
+    const i32x8_t bv = {b0[0], b0[1], b0[2], b0[3], b1[0], b1[1], b1[2], b1[3]};
+    acc = __builtin_amdgcn_mfma_scale_f32_32x32x64_f8f6f4(av, bv, acc, 0, 0, 0, 127, 0, 127);
+}
+
 template <int MODE, int OUT>
 __device__ inline void gemm_tile(const TileArgs& a, int tm, int tn, int b, char* smem) {
-    static_assert(MODE == kBF16 || MODE == kF16, "16-bit engine");
+    static_assert(MODE == kBF16 || MODE == kF16 || MODE == kF8, "16-bit or fp8 engine");
+    constexpr int ES = (MODE == kF8) ? 1 : 2;
+    constexpr int DU = RB / ES;
     const int lane = threadIdx.x & 63;
     const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     const int grp = wave >> 2, s = wave & 3;
     const int h = lane >> 5;
     const int m0 = tm * MT, n0 = tn * MT;
 
-    const char* Ab = (const char*)a.A + (int64_t)b * a.bsA * 2;
-    const char* Bb = (const char*)a.B + (int64_t)b * a.bsB * 2;
-    HalfLoader<true> la;
-    HalfLoader<false> lb;
-    la.init(Ab, extent_bytes(a.M, a.lda, a.K, 2), a.lda, a.M, a.K, m0, grp, s);
-    lb.init(Bb, extent_bytes(a.N, a.ldb, a.K, 2), a.ldb, a.N, a.K, n0, grp, s);
+    const char* Ab = (const char*)a.A + (int64_t)b * a.bsA * ES;
+    const char* Bb = (const char*)a.B + (int64_t)b * a.bsB * ES;
+    HalfLoader<true, ES> la;
+    HalfLoader<false, ES> lb;
+    la.init(Ab, extent_bytes(a.M, a.lda, a.K, ES), a.lda, a.M, a.K, m0, grp, s);
+    lb.init(Bb, extent_bytes(a.N, a.ldb, a.K, ES), a.ldb, a.N, a.K, n0, grp, s);
 
     char* const sA = smem;
     char* const sB = smem + NSA * STAGE;
@@ -111,7 +123,9 @@ __device__ inline void gemm_tile(const TileArgs& a, int tm, int tn, int b, char*
     // Fragment addresses.  A row 128*grp + 32i + lane%32 and B row
     // 64s + 32j + lane%32 both have swizzle (lane&7) ^ ((lane>>3)&3) ^ 4*(i|j & 1),
     // so chunk 2ks + h sits in slot y ^ c with c = 2ks ^ 4*(i|j & 1): four
-    // per-lane offsets serve every fragment; the rest is uniform.
+    // per-lane offsets serve every fragment; the rest is uniform.  (fp8: the
+    // 32-byte operand of lane group h for K-step s is chunks 4s + h and
+    // 4s + 2 + h, the chunks of 16-bit K-steps 2s and 2s + 1, in A and in B.)
     const int y = (h ^ (lane & 7) ^ ((lane >> 3) & 3)) & 7;
     int P[4];
 #pragma unroll
@@ -163,12 +177,19 @@ __device__ inline void gemm_tile(const TileArgs& a, int tm, int tn, int b, char*
             // ---- compute window
             if (half == 1) la.issue(kt + 2, stA(kt));  // this group's half of A is consumed
             __builtin_amdgcn_s_setprio(1);
-#pragma unroll
-            for (int k2 = 0; k2 < 2; ++k2)
+            if constexpr (MODE == kF8) {
 #pragma unroll
                 for (int i = 0; i < TI; ++i)
 #pragma unroll
-                    for (int j = 0; j < TJ; ++j) v2::mfma16<MODE>(fa[k2][i], fb[k2][j], acc[i][j]);
+                    for (int j = 0; j < TJ; ++j) mfma_f8(fa[0][i], fa[1][i], fb[0][j], fb[1][j], acc[i][j]);
+            } else {
+#pragma unroll
+                for (int k2 = 0; k2 < 2; ++k2)
+#pragma unroll
+                    for (int i = 0; i < TI; ++i)
+#pragma unroll
+                        for (int j = 0; j < TJ; ++j) v2::mfma16<MODE>(fa[k2][i], fb[k2][j], acc[i][j]);
+            }
             __builtin_amdgcn_s_setprio(0);
             __builtin_amdgcn_sched_barrier(0);
             if (half == 1) wait_vmcnt<2 * NLD>();
@@ -179,6 +200,12 @@ __device__ inline void gemm_tile(const TileArgs& a, int tm, int tn, int b, char*
     // before the workgroup's LDS is released
     wait_vmcnt<0>();
 
+    float alpha = a.alpha;
+    if constexpr (MODE == kF8) {  // per-tensor quantisation scales
+        const float sa = a.scaleA ? *a.scaleA : 1.0f;
+        const float sb = a.scaleB ? *a.scaleB : 1.0f;
+        alpha = alpha / (sa * sb);
+    }
     const char* Cb = (const char*)a.C + (int64_t)b * a.bsC * ((OUT == kOutF32) ? 4 : 2);
     char* Db = (char*)a.D + (int64_t)b * a.bsD * ((OUT == kOutF32) ? 4 : 2);
 #pragma unroll
@@ -191,7 +218,7 @@ __device__ inline void gemm_tile(const TileArgs& a, int tm, int tn, int b, char*
                 float v[16];
 #pragma unroll
                 for (int e = 0; e < 16; ++e) v[e] = acc[i][j][e];
-                store16<OUT>(a, Cb, Db, m, n, v, a.alpha, a.beta);
+                store16<OUT>(a, Cb, Db, m, n, v, alpha, a.beta);
             }
         }
     }

@@ -300,6 +300,34 @@ def test_fp8_quantize_bit_exact_and_gemm(cuda):
     assert err_vs_fp32_inputs < 0.1
 
 
+@pytest.mark.parametrize("tile", [(256, 256), (128, 128), (64, 64)])
+@pytest.mark.parametrize("out", ["f32", "bf16"])
+def test_fp8_tiles_edges(cuda, tile, out):
+    """FP8 GEMM on every engine: M, N not multiples of any tile and K = 208
+    (a partial K-tile of 80), operands quantised by the oracle's encoder."""
+    M, N, K = 328, 272, 208
+    _say(f"case fp8 tile={tile} out={out}")
+    rng = np.random.default_rng(21)
+    A = rng.standard_normal(M * K).astype(np.float32)   # K-contiguous: (m, k) at k + m*K
+    B = rng.standard_normal(N * K).astype(np.float32)
+    sa, sb = np.float32(oracle.fp8_scale(np.abs(A).max())), np.float32(oracle.fp8_scale(np.abs(B).max()))
+    qa, qb = oracle.f32_to_e4m3_bits(A, sa), oracle.f32_to_e4m3_bits(B, sb)
+    dt = torch.float32 if out == "f32" else torch.bfloat16
+    D = tl.gemm("TN", M, N, K, torch.from_numpy(qa).to(cuda), torch.from_numpy(qb).to(cuda), lda=K, ldb=K,
+                scaleA=torch.tensor([sa], device=cuda), scaleB=torch.tensor([sb], device=cuda), dtype_out=dt,
+                tile=tile)
+    torch.cuda.synchronize()
+    deq_a = oracle.e4m3_bits_to_f32(qa).astype(np.float64) / sa
+    deq_b = oracle.e4m3_bits_to_f32(qb).astype(np.float64) / sb
+    ref = oracle.gemm_f64("TN", M, N, K, 1.0, deq_a, K, deq_b, K)
+    got = D.float().cpu().numpy()
+    if out == "f32":
+        assert rel_fro(got, ref) <= 5e-5
+    else:
+        bad, worst = elem_ok(got, ref)
+        assert bad == 0, worst
+
+
 @pytest.mark.parametrize("layout", LAYOUTS)
 def test_fp64_oracle_product(cuda, layout):
     M, N, K = 130, 70, 90
