@@ -114,6 +114,21 @@ int ffgemm_query_tile(const ffgemm_problem *p, ffgemm_tile *out);
 int ffgemm_run(const ffgemm_problem *p, const void *A, const void *B, const void *C, void *D,
                const ffgemm_tile *tile, void *stream);
 
+/* Device workspace (bytes) that ffgemm_run_ws can use for this problem; 0
+ * when the problem has no workspace path.  Today only bf16x3: with a
+ * workspace the fp32 operands are split once into bf16 hi/lo planes
+ * ((M + N) rows x 3*roundup(K, 64) bf16 per batch) and the three products run
+ * as one bf16 MFMA GEMM over K' = 3*roundup(K, 64); without one they are
+ * split in registers inside the GEMM.  No counterpart in the reference (its
+ * precision forms stop at fp32). */
+int ffgemm_workspace_size(const ffgemm_problem *p, size_t *bytes);
+
+/* ffgemm_run with a caller-owned device workspace (256-byte aligned).  Falls
+ * back to ffgemm_run when the workspace is NULL or smaller than
+ * ffgemm_workspace_size reports.  Calls on one workspace must not overlap. */
+int ffgemm_run_ws(const ffgemm_problem *p, const void *A, const void *B, const void *C, void *D,
+                  const ffgemm_tile *tile, void *workspace, size_t workspace_bytes, void *stream);
+
 /* The reference kernel's argument list, in its kernarg order
  * D, C, A, B, alpha, beta, ldd, bsD, ldc, bsC, lda, bsA, ldb, bsB
  * (src/gemmMfma.cpp:106-119, src/gemmMfmaKernelWriter.h:323-338), TN layout,

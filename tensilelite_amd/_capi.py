@@ -64,6 +64,9 @@ def lib() -> ctypes.CDLL:
     L.ffgemm_device_cu_count.restype = i32
     L.ffgemm_query_tile.argtypes = [PP, PT]; L.ffgemm_query_tile.restype = i32
     L.ffgemm_run.argtypes = [PP, vp, vp, vp, vp, PT, vp]; L.ffgemm_run.restype = i32
+    L.ffgemm_workspace_size.argtypes = [PP, ctypes.POINTER(ctypes.c_size_t)]
+    L.ffgemm_workspace_size.restype = i32
+    L.ffgemm_run_ws.argtypes = [PP, vp, vp, vp, vp, PT, vp, ctypes.c_size_t, vp]; L.ffgemm_run_ws.restype = i32
     u32 = ctypes.c_uint32
     L.ffgemm_run_tn_kernarg.argtypes = [vp, vp, vp, vp, f32, f32] + [u32] * 12 + [i32, PT, vp]
     L.ffgemm_run_tn_kernarg.restype = i32

@@ -70,6 +70,43 @@ __global__ void fp8_quant_kernel(const float* __restrict__ x, int64_t rows, int6
     }
 }
 
+// bf16x3 operand planes (Henry, Tang & Heinecke 2019): for element (r, kk) of
+// a rows x k fp32 matrix (trans_x: at kk + r*ldx, else r + kk*ldx),
+// hi = rne_bf16(x) and lo = rne_bf16(x - hi).  Row r of the output holds three
+// K-segments of kp elements, out[r*3kp + s*kp + kk] = (order >> s) & 1 ? lo : hi,
+// zero for kk in [k, kp), so hi.hi + hi.lo + lo.hi of two operands is one bf16
+// GEMM over K = 3kp.  32 x 32 tiles go through LDS so reads and writes are
+// coalesced whatever the input layout; blockIdx.z is the batch.
+__global__ void x3_split_kernel(const float* __restrict__ x, int64_t rows, int64_t k, int64_t ldx,
+                                int64_t bsx, int trans_x, uint16_t* __restrict__ out, int64_t kp, int order) {
+    __shared__ float tile[32][33];
+    const int64_t r0 = (int64_t)blockIdx.y * 32, k0 = (int64_t)blockIdx.x * 32;
+    const int tx = threadIdx.x & 31, ty = threadIdx.x >> 5;  // 32 x 8
+    const float* xb = x + (int64_t)blockIdx.z * bsx;
+    uint16_t* ob = out + (int64_t)blockIdx.z * rows * 3 * kp;
+    for (int yy = ty; yy < 32; yy += 8) {
+        int64_t r, kk;
+        if (trans_x) { r = r0 + yy; kk = k0 + tx; } else { r = r0 + tx; kk = k0 + yy; }
+        float v = 0.0f;
+        if (r < rows && kk < k) v = trans_x ? xb[r * ldx + kk] : xb[kk * ldx + r];
+        if (trans_x) tile[yy][tx] = v; else tile[tx][yy] = v;
+    }
+    __syncthreads();
+    for (int yy = ty; yy < 32; yy += 8) {
+        const int64_t r = r0 + yy, kk = k0 + tx;
+        if (r < rows && kk < kp) {
+            const float v = tile[yy][tx];
+            const __bf16 hb = (__bf16)v;
+            const __bf16 lb = (__bf16)(v - (float)hb);
+            const uint16_t hi = __builtin_bit_cast(uint16_t, hb), lo = __builtin_bit_cast(uint16_t, lb);
+            uint16_t* o = ob + r * 3 * kp + kk;
+            o[0] = (order & 1) ? lo : hi;
+            o[kp] = (order & 2) ? lo : hi;
+            o[2 * kp] = (order & 4) ? lo : hi;
+        }
+    }
+}
+
 __global__ void convert_kernel(const float* __restrict__ src, void* __restrict__ dst, int64_t n, int to) {
     for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
         float v = src[i];

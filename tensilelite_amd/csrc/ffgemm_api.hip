@@ -312,6 +312,70 @@ int ffgemm_run(const ffgemm_problem* p, const void* A, const void* B, const void
     return launch(L, (hipStream_t)stream);
 }
 
+}  // extern "C"
+
+// bf16x3 through bf16 hi/lo planes in a workspace: K padded to the 64-element
+// K-tile; A planes (M rows x 3kp: hi, hi, lo) then B planes (N rows: hi, lo, hi).
+namespace {
+int64_t x3_kp(int64_t K) { return cdiv(K, 64) * 64; }
+int64_t round256(int64_t b) { return (b + 255) / 256 * 256; }
+bool x3_planes_apply(const ffgemm_problem* p) {
+    if (p->dtype != FFGEMM_F32 || p->compute != FFGEMM_COMPUTE_BF16X3) return false;
+    if (p->M <= 0 || p->N <= 0 || p->K <= 0) return false;
+    const int64_t batch = p->batch > 0 ? p->batch : 1;
+    const int64_t row = 3 * x3_kp(p->K) * 2;  // bytes per plane row
+    // plane extents stay within 31-bit buffer offsets; split grid limits
+    return p->M * row < (1ll << 31) && p->N * row < (1ll << 31) && cdiv(std::max(p->M, p->N), 32) <= 65535 &&
+           batch <= 65535;
+}
+int64_t x3_ws_bytes(const ffgemm_problem* p) {
+    const int64_t batch = p->batch > 0 ? p->batch : 1;
+    const int64_t row = 3 * x3_kp(p->K) * 2;
+    return round256(p->M * row * batch) + round256(p->N * row * batch);
+}
+}  // namespace
+
+extern "C" {
+
+int ffgemm_workspace_size(const ffgemm_problem* p, size_t* bytes) {
+    if (!p || !bytes) return fail(FFGEMM_ERR, "null pointer");
+    *bytes = x3_planes_apply(p) ? (size_t)x3_ws_bytes(p) : 0;
+    return FFGEMM_SUCCESS;
+}
+
+int ffgemm_run_ws(const ffgemm_problem* p, const void* A, const void* B, const void* C, void* D,
+                  const ffgemm_tile* tile, void* workspace, size_t workspace_bytes, void* stream) {
+    if (!p) return fail(FFGEMM_ERR, "null problem");
+    if (!x3_planes_apply(p) || !workspace || workspace_bytes < (size_t)x3_ws_bytes(p))
+        return ffgemm_run(p, A, B, C, D, tile, stream);
+    if ((uintptr_t)workspace % 256) return fail(FFGEMM_ERR, "workspace must be 256-byte aligned");
+    Launch L;
+    int r = prepare(p, A, B, C, D, nullptr, L);  // the fp32 problem's own checks
+    if (r != FFGEMM_SUCCESS) return r;
+    hipStream_t s = (hipStream_t)stream;
+    const int64_t batch = p->batch > 0 ? p->batch : 1;
+    const int64_t kp = x3_kp(p->K);
+    uint16_t* wa = (uint16_t*)workspace;
+    uint16_t* wb = (uint16_t*)((char*)workspace + round256(p->M * 3 * kp * 2 * batch));
+    const bool aK = p->layout >= FFGEMM_TN, bK = (p->layout & 1) == 0;
+    hipLaunchKernelGGL(x3_split_kernel, dim3((unsigned)(kp / 32), (unsigned)cdiv(p->M, 32), (unsigned)batch),
+                       dim3(256), 0, s, (const float*)A, p->M, p->K, p->lda, p->bsA, (int)aK, wa, kp, 4);
+    hipLaunchKernelGGL(x3_split_kernel, dim3((unsigned)(kp / 32), (unsigned)cdiv(p->N, 32), (unsigned)batch),
+                       dim3(256), 0, s, (const float*)B, p->N, p->K, p->ldb, p->bsB, (int)bK, wb, kp, 2);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return hip_fail(e, "bf16x3 split launch");
+    ffgemm_problem q = *p;
+    q.layout = FFGEMM_TN;
+    q.dtype = FFGEMM_BF16;
+    q.dtype_out = FFGEMM_F32;
+    q.compute = FFGEMM_COMPUTE_DEFAULT;
+    q.K = 3 * kp;
+    q.lda = q.ldb = 3 * kp;
+    q.bsA = p->M * 3 * kp;
+    q.bsB = p->N * 3 * kp;
+    return ffgemm_run(&q, wa, wb, C, D, tile, stream);
+}
+
 int ffgemm_run_tn_kernarg(void* D, const void* C, const void* A, const void* B, float alpha, float beta,
                           uint32_t ldd, uint32_t bsD, uint32_t ldc, uint32_t bsC, uint32_t lda, uint32_t bsA,
                           uint32_t ldb, uint32_t bsB, uint32_t M, uint32_t N, uint32_t K, uint32_t batch,

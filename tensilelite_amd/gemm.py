@@ -71,11 +71,29 @@ def make_tile(tile) -> C.Tile | None:
     return t
 
 
-def run(problem: C.Problem, A, B, Cm, D, tile=None, stream=None) -> None:
-    """ffgemm_run on device buffers (tensors or raw pointers)."""
+def workspace_size(problem: C.Problem) -> int:
+    """Device bytes ffgemm_run_ws can use for this problem (0: no workspace path)."""
+    n = ctypes.c_size_t()
+    C.check(C.lib().ffgemm_workspace_size(ctypes.byref(problem), ctypes.byref(n)))
+    return n.value
+
+
+def run(problem: C.Problem, A, B, Cm, D, tile=None, stream=None, workspace="auto") -> None:
+    """ffgemm_run on device buffers (tensors or raw pointers).
+
+    workspace: "auto" allocates what ffgemm_workspace_size asks for (from
+    torch's caching allocator) and runs ffgemm_run_ws; None runs ffgemm_run;
+    a uint8 device tensor is used as given."""
     t = make_tile(tile)
-    C.check(C.lib().ffgemm_run(ctypes.byref(problem), _ptr(A), _ptr(B), _ptr(Cm), _ptr(D),
-                               ctypes.byref(t) if t is not None else None, _stream(stream)))
+    tp = ctypes.byref(t) if t is not None else None
+    if isinstance(workspace, str):
+        n = workspace_size(problem)
+        workspace = torch.empty(n, dtype=torch.uint8, device=torch.device("cuda", torch.cuda.current_device())) if n else None
+    if workspace is None:
+        C.check(C.lib().ffgemm_run(ctypes.byref(problem), _ptr(A), _ptr(B), _ptr(Cm), _ptr(D), tp, _stream(stream)))
+    else:
+        C.check(C.lib().ffgemm_run_ws(ctypes.byref(problem), _ptr(A), _ptr(B), _ptr(Cm), _ptr(D), tp,
+                                      _ptr(workspace), workspace.numel(), _stream(stream)))
 
 
 def query_tile(problem: C.Problem, tile=None) -> C.Tile:
@@ -86,11 +104,13 @@ def query_tile(problem: C.Problem, tile=None) -> C.Tile:
 
 def gemm(layout, M, N, K, A, B, Cm=None, D=None, *, lda=None, ldb=None, ldc=None, ldd=None,
          alpha=1.0, beta=0.0, batch=1, bsA=0, bsB=0, bsC=0, bsD=0, compute=C.COMPUTE_DEFAULT,
-         round=C.ROUND_RNE, dtype_out=None, scaleA=None, scaleB=None, tile=None, stream=None):
+         round=C.ROUND_RNE, dtype_out=None, scaleA=None, scaleB=None, tile=None, stream=None,
+         workspace="auto"):
     """Column-major D = alpha*op(A)*op(B) + beta*C on flat device tensors.
 
     A, B, C, D are 1-D (or any contiguous) device tensors holding the
-    column-major matrices; D is allocated when not given.
+    column-major matrices; D is allocated when not given.  ``workspace`` as in
+    :func:`run`.
     """
     ta, tb = layout[0] == "T", layout[1] == "T"
     lda = lda or (K if ta else M)
@@ -104,7 +124,7 @@ def gemm(layout, M, N, K, A, B, Cm=None, D=None, *, lda=None, ldb=None, ldc=None
         D = torch.empty(n, dtype=out_dtype, device=A.device)
     p = make_problem(layout, in_code, M, N, K, lda, ldb, ldc, ldd, alpha, beta, batch, bsA, bsB, bsC,
                      bsD, _DT[out_dtype], compute, round, scaleA, scaleB)
-    run(p, A, B, Cm, D, tile, stream)
+    run(p, A, B, Cm, D, tile, stream, workspace)
     return D
 
 

@@ -95,3 +95,18 @@ def test_grouped_plan_rejects_empty_group():
     h = ctypes.c_void_p()
     st = C.lib().ffgemm_grouped_plan_create(ctypes.byref(h), 0, None, None, None, None, None, None)
     assert st == C.ERR
+
+
+def test_workspace_size_bf16x3_planes():
+    """bf16x3 asks for hi/lo planes of (M + N) rows x 3*roundup(K, 64) bf16 per
+    batch (each operand rounded to 256 bytes); other forms need none."""
+    M, N, K = 200, 264, 132
+    p = tl.make_problem("NN", C.F32, M, N, K, M, K, M, M, compute=C.COMPUTE_BF16X3)
+    kp = 192
+    assert tl.workspace_size(p) == ((M * 3 * kp * 2 + 255) // 256 + (N * 3 * kp * 2 + 255) // 256) * 256
+    p.batch = 3
+    assert tl.workspace_size(p) == ((3 * M * 3 * kp * 2 + 255) // 256 + (3 * N * 3 * kp * 2 + 255) // 256) * 256
+    q = tl.make_problem("NN", C.F32, M, N, K, M, K, M, M)
+    assert tl.workspace_size(q) == 0
+    r = tl.make_problem("TN", C.BF16, M, N, K, K, K, M, M)
+    assert tl.workspace_size(r) == 0

@@ -91,17 +91,19 @@ def test_16bit_ragged_m_n_padded_ld(cuda, name, layout, tile):
 
 @pytest.mark.parametrize("layout", LAYOUTS)
 @pytest.mark.parametrize("tile", [(256, 256), (128, 128), (64, 64)])
-@pytest.mark.parametrize("compute", ["f32", "bf16x3"])
+@pytest.mark.parametrize("compute", ["f32", "bf16x3", "bf16x3_regsplit"])
 def test_fp32_accuracy_forms(cuda, layout, tile, compute):
+    """fp32 MFMA and both bf16x3 forms (hi/lo planes in a workspace, or the
+    split in registers) within 1e-5 of FP64."""
     M, N, K = 200, 264, 132
     _say(f"case {compute} {layout} tile={tile}")
     rng = np.random.default_rng(2)
     (ra, ca), (rb, cb) = op_dims(layout, M, N, K)
     A = colmajor(rng, ra, ca, kind="normal")
     B = colmajor(rng, rb, cb, kind="normal")
-    comp = C.COMPUTE_BF16X3 if compute == "bf16x3" else C.COMPUTE_DEFAULT
+    comp = C.COMPUTE_DEFAULT if compute == "f32" else C.COMPUTE_BF16X3
     D = tl.gemm(layout, M, N, K, torch.from_numpy(A).to(cuda), torch.from_numpy(B).to(cuda),
-                compute=comp, tile=tile)
+                compute=comp, tile=tile, workspace=None if compute == "bf16x3_regsplit" else "auto")
     torch.cuda.synchronize()
     err = rel_fro(D.cpu().numpy(), _f64_ref(layout, M, N, K, A, ra, B, rb))
     _say(f"rel_fro={err:.3e}")
@@ -135,7 +137,7 @@ def test_alpha_beta_padding(cuda, name):
     got = Dd.float().cpu().numpy()
     ref = _f64_ref("TN", M, N, K, Ar, lda, Br, ldb, alpha, beta, Cr, ldc, ldd)
     g, r = extract(got, M, N, ldd), extract(ref, M, N, ldd)
-    if name == "f32":
+    if name in ("f32", "x3"):
         assert rel_fro(g, r) <= 1e-5
     else:
         bad, worst = elem_ok(g, r)
@@ -182,7 +184,7 @@ def test_zero_sized_is_noop(cuda):
     assert torch.all(D == 3.0)
 
 
-@pytest.mark.parametrize("name", ["bf16", "f32"])
+@pytest.mark.parametrize("name", ["bf16", "f32", "x3"])
 def test_batched(cuda, name):
     M, N, K, batch = 136, 120, 96, 3
     _say(f"case batch {name}")
@@ -191,18 +193,19 @@ def test_batched(cuda, name):
     bsA, bsB, bsD = lda * M + 64, ldb * N + 128, ldd * N + 32
     A = np.concatenate([np.pad(colmajor(rng, K, M, lda), (0, 64)) for _ in range(batch)])
     B = np.concatenate([np.pad(colmajor(rng, K, N, ldb), (0, 128)) for _ in range(batch)])
-    if name == "f32":
+    if name in ("f32", "x3"):
         Ad, Bd, Ar, Br = torch.from_numpy(A).to(cuda), torch.from_numpy(B).to(cuda), A, B
     else:
         Ab, Bb = quantize_bits(A, name), quantize_bits(B, name)
         Ad, Bd, Ar, Br = _dev16(Ab, name, cuda), _dev16(Bb, name, cuda), bits_to_f32(Ab, name), bits_to_f32(Bb, name)
     D = tl.gemm("TN", M, N, K, Ad, Bd, batch=batch, bsA=bsA, bsB=bsB, bsD=bsD, lda=lda, ldb=ldb, ldd=ldd,
-                D=torch.zeros(bsD * batch, dtype=Ad.dtype, device=cuda))
+                D=torch.zeros(bsD * batch, dtype=Ad.dtype, device=cuda),
+                compute=C.COMPUTE_BF16X3 if name == "x3" else C.COMPUTE_DEFAULT)
     torch.cuda.synchronize()
     ref = _f64_ref("TN", M, N, K, Ar, lda, Br, ldb, ldd=ldd, batch=batch, bsA=bsA, bsB=bsB, bsD=bsD)
     g = extract(D.float().cpu().numpy(), M, N, ldd, batch, bsD)
     r = extract(ref, M, N, ldd, batch, bsD)
-    if name == "f32":
+    if name in ("f32", "x3"):
         assert rel_fro(g, r) <= 1e-5
     else:
         assert elem_ok(g, r)[0] == 0
