@@ -121,7 +121,7 @@ def main():
         "roofline": {"bound": "mfma", "achieved": round(tflops, 2), "peak": PEAK_TFLOPS["bf16"],
                      "unit": "TFLOP/s", "frac": round(tflops / PEAK_TFLOPS["bf16"], 4),
                      "traffic": profiled_traffic(), "traffic_source": TRAFFIC_PROFILE,
-                     "kernel": "ffgemm::v3::gemm_kernel<bf16,TN,%dx%d>" % (tile.mt_m, tile.mt_n),
+                     "kernel": "ffgemm::v4::gemm_kernel<bf16,TN,%dx%d>" % (tile.mt_m, tile.mt_n),
                      "algorithmic_flop_per_launch": flop, "wall_s": round(wall, 4)},
     }
 
