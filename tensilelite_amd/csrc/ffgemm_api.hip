@@ -64,12 +64,20 @@ bool tile_supported(int m, int n) {
     return false;
 }
 
-// Resident workgroups per CU for a 16-bit tile (VGPR/AGPR and LDS limited):
-// 256 x 128 fits twice only when both operands are K-contiguous (TN).
+// Resident workgroups per CU for a 16-bit tile (VGPR/AGPR and LDS limited),
+// as the pipelined engine's configuration declares them.
 int tile_occupancy(int m, int n, int layout) {
     if (m == 64 && n == 64) return 4;
-    if (m == 128 && n == 128) return 2;
-    if (m != n && layout == FFGEMM_TN) return 2;
+    const bool ak = layout >= FFGEMM_TN, bk = (layout & 1) == 0;
+#define FFGEMM_OCC(TM_, TN_)                                                               \
+    if (m == TM_ && n == TN_) {                                                            \
+        if (ak && bk) return v2::Cfg<TM_, TN_, true, true>::OCC;                           \
+        if (ak) return v2::Cfg<TM_, TN_, true, false>::OCC;                                \
+        if (bk) return v2::Cfg<TM_, TN_, false, true>::OCC;                                \
+        return v2::Cfg<TM_, TN_, false, false>::OCC;                                       \
+    }
+    FFGEMM_OCC(256, 128) FFGEMM_OCC(128, 256) FFGEMM_OCC(128, 128)
+#undef FFGEMM_OCC
     return 1;
 }
 

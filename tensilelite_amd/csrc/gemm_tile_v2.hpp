@@ -39,7 +39,12 @@ struct Cfg {
     static constexpr int RB = 64;                      // LDS bytes per row per stage
     static constexpr int DU = RB / 2;                  // K per stage (16-bit)
     static constexpr int RS = DU / 16;                 // MFMA read steps per stage
-    static constexpr int OCC = kBig ? 1 : ((kSmall || (AK && BK)) ? 2 : 1);
+    // two workgroups per CU where the registers allow it without spills: the
+    // 128 x 128 tiles, both-K-contiguous tiles, and the two shapes whose one
+    // M/N-contiguous operand is the 128-row side
+    static constexpr bool kTwoFit = kSmall || (AK && BK) || (MT_M == 128 && MT_N == 256 && !AK && BK) ||
+                                    (MT_M == 256 && MT_N == 128 && AK && !BK);
+    static constexpr int OCC = kBig ? 1 : (kTwoFit ? 2 : 1);
     static constexpr int NS = kBig ? 4 : ((OCC == 2 && !kSmall) ? 3 : 4);
     static constexpr int PF = NS - 1;                  // K-tiles in flight
 };

@@ -93,14 +93,6 @@ def bench_config(device, steps=20, tile=None):
     for _ in range(3):
         plan.run()
     torch.cuda.synchronize()
-    s = torch.cuda.current_stream()
-    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
-    e0.record(s)
-    for _ in range(steps):
-        plan.run()
-    e1.record(s)
-    torch.cuda.synchronize()
-    ms_eager = e0.elapsed_time(e1) / steps
     graph = torch.cuda.CUDAGraph()
     side = torch.cuda.Stream()
     side.wait_stream(torch.cuda.current_stream())
@@ -112,12 +104,24 @@ def bench_config(device, steps=20, tile=None):
         plan.run()
     graph.replay()
     torch.cuda.synchronize()
-    e0.record(s)
-    for _ in range(steps):
-        graph.replay()
-    e1.record(s)
-    torch.cuda.synchronize()
-    ms_graph = e0.elapsed_time(e1) / steps
+    s = torch.cuda.current_stream()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+
+    def timed(fn):
+        e0.record(s)
+        for _ in range(steps):
+            fn()
+        e1.record(s)
+        torch.cuda.synchronize()
+        return e0.elapsed_time(e1) / steps
+
+    # eager and graph blocks interleaved (the clock drifts with load), median of each
+    eager, graphed = [], []
+    for _ in range(3):
+        eager.append(timed(plan.run))
+        graphed.append(timed(graph.replay))
+    ms_eager = sorted(eager)[1]
+    ms_graph = sorted(graphed)[1]
     # per-problem roofline: max(MFMA time, HBM time)
     t_roof = sum(max(2.0 * p.M * p.N * p.K / 2500e12, 2.0 * (p.M * p.K + p.K * p.N + p.M * p.N) / 8e12)
                  for p in probs)
