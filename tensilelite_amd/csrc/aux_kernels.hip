@@ -73,37 +73,65 @@ __global__ void fp8_quant_kernel(const float* __restrict__ x, int64_t rows, int6
 // bf16x3 operand planes (Henry, Tang & Heinecke 2019): for element (r, kk) of
 // a rows x k fp32 matrix (trans_x: at kk + r*ldx, else r + kk*ldx),
 // hi = rne_bf16(x) and lo = rne_bf16(x - hi).  Row r of the output holds three
-// K-segments of kp elements, out[r*3kp + s*kp + kk] = (order >> s) & 1 ? lo : hi,
-// zero for kk in [k, kp), so hi.hi + hi.lo + lo.hi of two operands is one bf16
-// GEMM over K = 3kp.  32 x 32 tiles go through LDS so reads and writes are
-// coalesced whatever the input layout; blockIdx.z is the batch.
-__global__ void x3_split_kernel(const float* __restrict__ x, int64_t rows, int64_t k, int64_t ldx,
-                                int64_t bsx, int trans_x, uint16_t* __restrict__ out, int64_t kp, int order) {
-    __shared__ float tile[32][33];
-    const int64_t r0 = (int64_t)blockIdx.y * 32, k0 = (int64_t)blockIdx.x * 32;
-    const int tx = threadIdx.x & 31, ty = threadIdx.x >> 5;  // 32 x 8
+// K-segments of kp elements (kp a multiple of 64), out[r*3kp + s*kp + kk] =
+// (order >> s) & 1 ? lo : hi, zero for kk in [k, kp), so hi.hi + hi.lo + lo.hi
+// of two operands is one bf16 GEMM over K = 3kp.  A block covers 64 rows x
+// 64 kk; each thread converts 8 consecutive kk of one row and stores them as
+// one 16-byte vector per segment.  M-contiguous input goes through an LDS
+// transpose so the fp32 reads stay coalesced.  blockIdx.z is the batch.
+__device__ inline void x3_store8(const float* v, uint16_t* o, int64_t kp, int order) {
+    uint32_t hw[4], lw[4];
+#pragma unroll
+    for (int e = 0; e < 4; ++e) {
+        const __bf16 h0 = (__bf16)v[2 * e], h1 = (__bf16)v[2 * e + 1];
+        const __bf16 l0 = (__bf16)(v[2 * e] - (float)h0), l1 = (__bf16)(v[2 * e + 1] - (float)h1);
+        hw[e] = (uint32_t)__builtin_bit_cast(uint16_t, h0) | ((uint32_t)__builtin_bit_cast(uint16_t, h1) << 16);
+        lw[e] = (uint32_t)__builtin_bit_cast(uint16_t, l0) | ((uint32_t)__builtin_bit_cast(uint16_t, l1) << 16);
+    }
+    const i32x4_t hv = {(int)hw[0], (int)hw[1], (int)hw[2], (int)hw[3]};
+    const i32x4_t lv = {(int)lw[0], (int)lw[1], (int)lw[2], (int)lw[3]};
+    *(i32x4_t*)o = (order & 1) ? lv : hv;
+    *(i32x4_t*)(o + kp) = (order & 2) ? lv : hv;
+    *(i32x4_t*)(o + 2 * kp) = (order & 4) ? lv : hv;
+}
+
+__global__ void __launch_bounds__(256) x3_split_kernel(const float* __restrict__ x, int64_t rows, int64_t k,
+                                                       int64_t ldx, int64_t bsx, int trans_x,
+                                                       uint16_t* __restrict__ out, int64_t kp, int order) {
+    __shared__ float tile[64][65];
+    const int64_t r0 = (int64_t)blockIdx.y * 64, k0 = (int64_t)blockIdx.x * 64;
     const float* xb = x + (int64_t)blockIdx.z * bsx;
     uint16_t* ob = out + (int64_t)blockIdx.z * rows * 3 * kp;
-    for (int yy = ty; yy < 32; yy += 8) {
-        int64_t r, kk;
-        if (trans_x) { r = r0 + yy; kk = k0 + tx; } else { r = r0 + tx; kk = k0 + yy; }
-        float v = 0.0f;
-        if (r < rows && kk < k) v = trans_x ? xb[r * ldx + kk] : xb[kk * ldx + r];
-        if (trans_x) tile[yy][tx] = v; else tile[tx][yy] = v;
-    }
-    __syncthreads();
-    for (int yy = ty; yy < 32; yy += 8) {
-        const int64_t r = r0 + yy, kk = k0 + tx;
-        if (r < rows && kk < kp) {
-            const float v = tile[yy][tx];
-            const __bf16 hb = (__bf16)v;
-            const __bf16 lb = (__bf16)(v - (float)hb);
-            const uint16_t hi = __builtin_bit_cast(uint16_t, hb), lo = __builtin_bit_cast(uint16_t, lb);
-            uint16_t* o = ob + r * 3 * kp + kk;
-            o[0] = (order & 1) ? lo : hi;
-            o[kp] = (order & 2) ? lo : hi;
-            o[2 * kp] = (order & 4) ? lo : hi;
+    const int t = threadIdx.x;
+    const int kc = t & 7;  // this thread's 8-element K chunk
+    if (!trans_x) {
+        // M-contiguous: 64 kk columns of 64 rows, read along the rows
+        for (int i = t; i < 64 * 16; i += 256) {
+            const int kk = i >> 4, rq = (i & 15) * 4;
+            const int64_t kg = k0 + kk;
+#pragma unroll
+            for (int e = 0; e < 4; ++e) {
+                const int64_t rg = r0 + rq + e;
+                tile[rq + e][kk] = (rg < rows && kg < k) ? xb[kg * ldx + rg] : 0.0f;
+            }
         }
+        __syncthreads();
+    }
+#pragma unroll
+    for (int pass = 0; pass < 2; ++pass) {
+        const int rl = (t >> 3) + 32 * pass;
+        const int64_t r = r0 + rl, kk = k0 + kc * 8;
+        if (r >= rows) continue;
+        float v[8];
+        if (trans_x) {
+            const float* src = xb + r * ldx + kk;
+#pragma unroll
+            for (int e = 0; e < 8; ++e) v[e] = (kk + e < k) ? src[e] : 0.0f;
+        } else {
+#pragma unroll
+            for (int e = 0; e < 8; ++e) v[e] = tile[rl][kc * 8 + e];
+        }
+        x3_store8(v, ob + r * 3 * kp + kk, kp, order);
     }
 }
 

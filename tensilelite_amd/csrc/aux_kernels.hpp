@@ -21,8 +21,9 @@ __global__ void fp8_scale_kernel(const uint32_t* __restrict__ amax_bits, float* 
 __global__ void fp8_quant_kernel(const float* __restrict__ x, int64_t rows, int64_t k, int64_t ldx,
                                  int trans_x, uint8_t* __restrict__ q, int64_t ldq,
                                  const float* __restrict__ scale);
-__global__ void x3_split_kernel(const float* __restrict__ x, int64_t rows, int64_t k, int64_t ldx,
-                                int64_t bsx, int trans_x, uint16_t* __restrict__ out, int64_t kp, int order);
+__global__ void __launch_bounds__(256) x3_split_kernel(const float* __restrict__ x, int64_t rows, int64_t k,
+                                                       int64_t ldx, int64_t bsx, int trans_x,
+                                                       uint16_t* __restrict__ out, int64_t kp, int order);
 __global__ void convert_kernel(const float* __restrict__ src, void* __restrict__ dst, int64_t n, int to);
 __global__ void __launch_bounds__(256) gemm_f64_kernel(F64Args a);
 

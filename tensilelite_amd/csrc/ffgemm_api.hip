@@ -333,7 +333,7 @@ bool x3_planes_apply(const ffgemm_problem* p) {
     const int64_t batch = p->batch > 0 ? p->batch : 1;
     const int64_t row = 3 * x3_kp(p->K) * 2;  // bytes per plane row
     // plane extents stay within 31-bit buffer offsets; split grid limits
-    return p->M * row < (1ll << 31) && p->N * row < (1ll << 31) && cdiv(std::max(p->M, p->N), 32) <= 65535 &&
+    return p->M * row < (1ll << 31) && p->N * row < (1ll << 31) && cdiv(std::max(p->M, p->N), 64) <= 65535 &&
            batch <= 65535;
 }
 int64_t x3_ws_bytes(const ffgemm_problem* p) {
@@ -366,9 +366,9 @@ int ffgemm_run_ws(const ffgemm_problem* p, const void* A, const void* B, const v
     uint16_t* wa = (uint16_t*)workspace;
     uint16_t* wb = (uint16_t*)((char*)workspace + round256(p->M * 3 * kp * 2 * batch));
     const bool aK = p->layout >= FFGEMM_TN, bK = (p->layout & 1) == 0;
-    hipLaunchKernelGGL(x3_split_kernel, dim3((unsigned)(kp / 32), (unsigned)cdiv(p->M, 32), (unsigned)batch),
+    hipLaunchKernelGGL(x3_split_kernel, dim3((unsigned)(kp / 64), (unsigned)cdiv(p->M, 64), (unsigned)batch),
                        dim3(256), 0, s, (const float*)A, p->M, p->K, p->lda, p->bsA, (int)aK, wa, kp, 4);
-    hipLaunchKernelGGL(x3_split_kernel, dim3((unsigned)(kp / 32), (unsigned)cdiv(p->N, 32), (unsigned)batch),
+    hipLaunchKernelGGL(x3_split_kernel, dim3((unsigned)(kp / 64), (unsigned)cdiv(p->N, 64), (unsigned)batch),
                        dim3(256), 0, s, (const float*)B, p->N, p->K, p->ldb, p->bsB, (int)bK, wb, kp, 2);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return hip_fail(e, "bf16x3 split launch");

@@ -118,6 +118,9 @@ protected:
     ffgemm_problem prob_{};
     ffgemm_tile tile_{};
     bool explicit_tile_ = false;
+    bool ws_checked_ = false;
+    void* ws_ = nullptr;
+    size_t ws_bytes_ = 0;
 
     E_ReturnState generateKernel() override {
         CmdArgs* ca = CmdArgs::Get();
@@ -176,8 +179,18 @@ protected:
 
     E_ReturnState launchKernel() override {
         const ffgemm_tile* t = explicit_tile_ ? &tile_ : nullptr;
-        return stream_->Launch([&](hipStream_t s) { return ffgemm_run(&prob_, dA, dB, dC, dD, t, s); });
+        if (!ws_checked_) {  // the library's workspace path (bf16x3 planes), allocated once
+            ws_checked_ = true;
+            if (ffgemm_workspace_size(&prob_, &ws_bytes_) == FFGEMM_SUCCESS && ws_bytes_ > 0) {
+                ws_ = RuntimeHip::GetInstance()->DevMalloc(ws_bytes_);
+                if (ws_) FF_LOG("workspace: %zu bytes", ws_bytes_);
+                else ws_bytes_ = 0;
+            }
+        }
+        return stream_->Launch(
+            [&](hipStream_t s) { return ffgemm_run_ws(&prob_, dA, dB, dC, dD, t, ws_, ws_bytes_, s); });
     }
+    ~GemmMfmaSolution() override { RuntimeHip::GetInstance()->DevFree(ws_); }
 
     // src/gemmMfma.cpp:348-421 plus the build's tolerances.  Returns RTN_WARN on
     // a mismatch so the process exit code reflects correctness.
