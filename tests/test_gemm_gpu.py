@@ -377,3 +377,63 @@ def test_4096_bf16_sampled_columns(cuda, layout, beta):
     colsum_ref = (opA.sum(axis=0) @ opB) + beta * C64.sum(axis=1)
     colsum = Dh.astype(np.float64).sum(axis=1)
     assert rel_fro(colsum, colsum_ref) < 2e-3
+
+
+def test_config1_fp32_nn_480x512x512_vs_reference_loop(cuda):
+    """The reference's own CPU case (fp32 NN 480 x 512 x 512, alpha 1, beta 0,
+    uniform[-1, 1] from seed 0) on the GPU, against the oracle's restatement of
+    the reference CPU GEMM (src/gemmMfma.cpp:325-346)."""
+    M, N, K = 480, 512, 512
+    _say("case config1 fp32 NN 480x512x512")
+    rng = np.random.default_rng(0)
+    A = rng.uniform(-1, 1, M * K).astype(np.float32)
+    B = rng.uniform(-1, 1, K * N).astype(np.float32)
+    D = tl.gemm("NN", M, N, K, torch.from_numpy(A).to(cuda), torch.from_numpy(B).to(cuda))
+    torch.cuda.synchronize()
+    ref = oracle.gemm_f32("NN", M, N, K, 1.0, A, M, B, K)
+    assert rel_fro(D.cpu().numpy(), ref) <= 1e-5
+    assert rel_fro(D.cpu().numpy(), _f64_ref("NN", M, N, K, A, M, B, K)) <= 1e-5
+
+
+def test_fp8_4096_sampled_columns(cuda):
+    """fp8 e4m3 TN at 4096^3 (device quantisation, fp32 out): 16 sampled output
+    columns against the FP64 product of the dequantised operands."""
+    S = 4096
+    _say("case fp8 4096 sampled")
+    g = torch.Generator(device="cpu").manual_seed(2)
+    A = torch.randn(S * S, generator=g).to(cuda)   # K-contiguous (k + m*K)
+    B = torch.randn(S * S, generator=g).to(cuda)
+    qa, sa = tl.fp8.quantize(A, rows=S, k=S, ldx=S, trans_x=True)
+    qb, sb = tl.fp8.quantize(B, rows=S, k=S, ldx=S, trans_x=True)
+    D = tl.gemm("TN", S, S, S, qa, qb, lda=S, ldb=S, scaleA=sa, scaleB=sb, dtype_out=torch.float32)
+    torch.cuda.synchronize()
+    cols = np.random.default_rng(1).choice(S, 16, replace=False)
+    deq_a = oracle.e4m3_bits_to_f32(qa.cpu().numpy()).astype(np.float64) / sa.item()
+    qb_h = qb.cpu().numpy().reshape(S, S)[cols]
+    deq_b = oracle.e4m3_bits_to_f32(qb_h.ravel()).astype(np.float64) / sb.item()
+    ref = oracle.gemm_f64("TN", S, len(cols), S, 1.0, deq_a, S, deq_b, S).reshape(len(cols), S)
+    got = D.cpu().numpy().reshape(S, S)[cols]
+    assert rel_fro(got, ref) <= 5e-5
+
+
+@pytest.mark.parametrize("compute", ["f32", "bf16x3"])
+def test_fp32_forms_8192_sampled_columns(cuda, compute):
+    """The precision configuration at full size (NN, 8192^3, normal(0, 1) from
+    seed 2): 8 sampled output columns within 1e-5 of FP64."""
+    S = 8192
+    _say(f"case {compute} 8192 sampled")
+    g = torch.Generator(device="cpu").manual_seed(2)
+    A = torch.randn(S * S, generator=g)
+    B = torch.randn(S * S, generator=g)
+    comp = C.COMPUTE_BF16X3 if compute == "bf16x3" else C.COMPUTE_DEFAULT
+    D = tl.gemm("NN", S, S, S, A.to(cuda), B.to(cuda), compute=comp)
+    torch.cuda.synchronize()
+    cols = np.random.default_rng(3).choice(S, 8, replace=False)
+    Bc = B.numpy().reshape(S, S)[cols]          # column n of B (K-contiguous) = row n of this view
+    At = A.numpy().reshape(S, S).T.astype(np.float64).ravel()   # op(A) K-contiguous: the oracle's TN form
+    ref = oracle.gemm_f64("TN", S, len(cols), S, 1.0, At, S, Bc.astype(np.float64).ravel(), S).reshape(len(cols), S)
+    del At
+    got = D.cpu().numpy().reshape(S, S)[cols]
+    err = rel_fro(got, ref)
+    _say(f"rel_fro={err:.3e}")
+    assert err <= 1e-5
