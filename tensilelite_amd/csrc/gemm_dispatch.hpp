@@ -12,16 +12,17 @@ namespace ffgemm {
 
 template <int MODE, bool AK, bool BK, int OUT>
 inline int launch_layout(int mt_m, int mt_n, const TileArgs& a, int grid, hipStream_t s) {
-    // bf16/fp16/fp8: 256 x 256 runs the eight-wave ping-pong engine (v4::) when
-    // both operands are K-contiguous; bf16/fp16: the split-ring engine (v3::) when only B is (with
-    // an M/N-contiguous B its second register ring would spill), other tiles of
-    // 128 and up the pipelined engine (v2::), the rest the first engine
+    // 256 x 256, by what measured fastest on MI355X at 4096^3: the eight-wave
+    // ping-pong engine (v4::) for TN and NT (M/N-contiguous operands through
+    // transposing LDS reads) and fp8; the split-ring engine (v3::) for NN; the
+    // pipelined 4-wave engine (v2::) for TT and for the other tiles of 128 and
+    // up; the first engine for the rest (and fp32 / bf16x3 / fp8 below 256).
     constexpr bool k16 = (MODE == kBF16 || MODE == kF16);
 #define FFGEMM_CASE(TM_, TN_)                                                                  \
     if (mt_m == TM_ && mt_n == TN_) {                                                          \
-        if constexpr ((k16 || MODE == kF8) && TM_ == 256 && TN_ == 256 && AK && BK)           \
-            hipLaunchKernelGGL((v4::gemm_kernel<MODE, OUT>), dim3(grid), dim3(v4::kThreads8), 0, s, a); \
-        else if constexpr (k16 && TM_ == 256 && TN_ == 256 && BK)                              \
+        if constexpr (TM_ == 256 && TN_ == 256 && ((k16 && AK == BK) || (MODE == kF8 && AK && BK))) \
+            hipLaunchKernelGGL((v4::gemm_kernel<MODE, AK, BK, OUT>), dim3(grid), dim3(v4::kThreads8), 0, s, a); \
+        else if constexpr (k16 && TM_ == 256 && TN_ == 256 && !AK && BK)                       \
             hipLaunchKernelGGL((v3::gemm_kernel<MODE, AK, BK, OUT>), dim3(grid), dim3(kThreads), 0, s, a); \
         else if constexpr (k16 && TM_ >= 128 && TN_ >= 128)                                    \
             hipLaunchKernelGGL((v2::gemm_kernel<MODE, AK, BK, TM_, TN_, OUT>), dim3(grid), dim3(kThreads), 0, s, a); \

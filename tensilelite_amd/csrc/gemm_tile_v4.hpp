@@ -1,5 +1,8 @@
 // Eight-wave ping-pong engine: 256 x 256 tiles of bf16 / fp16 / fp8 (OCP
-// e4m3) operands with both operands K-contiguous (TN, the reference's layout).
+// e4m3) operands.  K-contiguous operands (TN, the reference's layout) load
+// into 128-byte K rows; M/N-contiguous 16-bit operands (NT) load in their own
+// orientation and reach the MFMAs through transposing LDS reads
+// (ds_read_b64_tr_b16), so every load is a direct HBM -> LDS copy.
 //
 // Eight waves, two per SIMD.  The four waves of group 0 (waves 0-3) own
 // output rows 0-127 of the tile, group 1 (waves 4-7) rows 128-255; wave s of
@@ -87,6 +90,69 @@ struct HalfLoader {
     }
 };
 
+// Half (128 rows) of a 256-row M/N-contiguous 16-bit operand for one K-tile,
+// loaded straight into LDS in its own orientation: region [64 k][128 rows],
+// 256 bytes per k, 16-byte chunk c of k stored in slot c ^ 4*(k % 4), which
+// makes the transposing fragment reads below conflict-free.  Wave s of the
+// group fills k = 16s + 4q + lane/16 (q = 0..3), one 1 KiB instruction per q:
+// lane p of a k takes the 8 rows of chunk p ^ 4*(k % 4), coalesced 256 bytes
+// per k.  k beyond K reads zero; rows beyond the matrix only reach output rows
+// that are never stored.
+struct MNHalfLoader {
+    __amdgpu_buffer_rsrc_t rsrc;
+    uint32_t base;     // this lane's byte offset for kt = 0, q = 0
+    int32_t klim;      // K - (this lane's k for kt = 0, q = 0)
+    uint32_t qstride;  // 4 k
+    uint32_t kstride;  // 64 k
+    int32_t lds_off;   // this wave's first k, in bytes of the region
+
+    __device__ inline void init(const void* ptr, uint32_t extent, int32_t ld, int32_t K, int32_t row0, int s) {
+        rsrc = make_rsrc(ptr, extent);
+        const int lane = threadIdx.x & 63;
+        const int kr = 16 * s + (lane >> 4);
+        const int c = (lane & 15) ^ (4 * (lane >> 4));
+        base = (uint32_t)(((int64_t)kr * ld + row0 + 8 * c) * 2);
+        klim = K - kr;
+        qstride = (uint32_t)(4 * ld * 2);
+        kstride = (uint32_t)(64 * ld * 2);
+        lds_off = 16 * s * 256;
+    }
+    __device__ inline void issue(int kt, char* region) const {
+        char* dst = region + lds_off;
+#pragma unroll
+        for (int q = 0; q < NLD; ++q) {
+            const bool ok = kt * 64 + 4 * q < klim;
+            const uint32_t off = ok ? base + (uint32_t)kt * kstride + q * qstride : kOOB;
+            __builtin_amdgcn_raw_ptr_buffer_load_lds(rsrc, (lds_ptr_t)(dst + q * 1024), 16, off, 0, 0, 0);
+        }
+    }
+};
+
+typedef short i16x4_t __attribute__((ext_vector_type(4)));
+typedef __attribute__((address_space(3))) i16x4_t lds_i16x4_t;
+
+// ds_read_b64_tr_b16: within each 16-lane group, source lane s reads 4
+// consecutive rows at one k, and lane i receives element j of source lane
+// 4j + i/4 (position i % 4), i.e. 4 consecutive k of one row.
+__device__ inline i16x4_t read_tr16(const char* p) {
+    return __builtin_amdgcn_ds_read_tr16_b64_v4i16(
+        (lds_i16x4_t*)((__attribute__((address_space(3))) char*)(p)));
+}
+
+// Per-lane offset, inside an [64 k][128 rows] region, of the transposing read
+// that gives lane l the 4 k from k0 = 8*(l/32) (+ kb, a multiple of 4, added by
+// the caller as kb*256) of row `row` of the MFMA operand, where row = r0 +
+// (PERM ? perm_a_inv(l % 32) : l % 32): source lane s reads rows cs..cs+3 at
+// k = k0 + s/4.
+template <bool PERM>
+__device__ inline int tr_offset(int lane, int r0) {
+    const int G = lane >> 4, s = lane & 15, q = s & 3, gp = G & 1;
+    const int cs = PERM ? (16 * (q & 1) + 8 * gp + 4 * (q >> 1)) : (16 * gp + 4 * q);
+    const int k = 8 * (G >> 1) + (s >> 2);
+    const int b = (r0 + cs) * 2;  // byte of the row inside the 256-byte k line
+    return k * 256 + ((((b >> 4) ^ (4 * (k & 3))) & 15) << 4) + (b & 15);
+}
+
 __device__ inline void barrier() { asm volatile("s_barrier" ::: "memory"); }
 
 // OCP e4m3 x e4m3, 32x32x64, unit block scales (e8m0 127 = 1.0).
@@ -97,9 +163,10 @@ __device__ inline void mfma_f8(const i32x4_t& a0, const i32x4_t& a1, const i32x4
     acc = __builtin_amdgcn_mfma_scale_f32_32x32x64_f8f6f4(av, bv, acc, 0, 0, 0, 127, 0, 127);
 }
 
-template <int MODE, int OUT>
+template <int MODE, bool AK, bool BK, int OUT>
 __device__ inline void gemm_tile(const TileArgs& a, int tm, int tn, int b, char* smem) {
     static_assert(MODE == kBF16 || MODE == kF16 || MODE == kF8, "16-bit or fp8 engine");
+    static_assert((AK && BK) || MODE != kF8, "fp8 operands are K-contiguous");
     constexpr int ES = (MODE == kF8) ? 1 : 2;
     constexpr int DU = RB / ES;
     const int lane = threadIdx.x & 63;
@@ -110,10 +177,18 @@ __device__ inline void gemm_tile(const TileArgs& a, int tm, int tn, int b, char*
 
     const char* Ab = (const char*)a.A + (int64_t)b * a.bsA * ES;
     const char* Bb = (const char*)a.B + (int64_t)b * a.bsB * ES;
-    HalfLoader<true, ES> la;
-    HalfLoader<false, ES> lb;
-    la.init(Ab, extent_bytes(a.M, a.lda, a.K, ES), a.lda, a.M, a.K, m0, grp, s);
-    lb.init(Bb, extent_bytes(a.N, a.ldb, a.K, ES), a.ldb, a.N, a.K, n0, grp, s);
+    // K-contiguous operands: rows of 128 bytes of K; M/N-contiguous ones: the
+    // group halves of a stage are [64 k][128 rows] regions (MNHalfLoader)
+    using LA = std::conditional_t<AK, HalfLoader<true, ES>, MNHalfLoader>;
+    using LB = std::conditional_t<BK, HalfLoader<false, ES>, MNHalfLoader>;
+    LA la;
+    LB lb;
+    if constexpr (AK) la.init(Ab, extent_bytes(a.M, a.lda, a.K, ES), a.lda, a.M, a.K, m0, grp, s);
+    else la.init(Ab, extent_bytes(a.K, a.lda, a.M, 2), a.lda, a.K, m0 + 128 * grp, s);
+    if constexpr (BK) lb.init(Bb, extent_bytes(a.N, a.ldb, a.K, ES), a.ldb, a.N, a.K, n0, grp, s);
+    else lb.init(Bb, extent_bytes(a.K, a.ldb, a.N, 2), a.ldb, a.K, n0 + 128 * grp, s);
+    // where this group's loads go inside a stage (K-contiguous loaders add it themselves)
+    const int lhalf_a = AK ? 0 : 16384 * grp, lhalf_b = BK ? 0 : 16384 * grp;
 
     char* const sA = smem;
     char* const sB = smem + NSA * STAGE;
@@ -131,6 +206,19 @@ __device__ inline void gemm_tile(const TileArgs& a, int tm, int tn, int b, char*
 #pragma unroll
     for (int c = 0; c < 4; ++c) P[c] = (lane & 31) * RB + ((y ^ (2 * c)) << 4);
     const int aoff = 128 * grp * RB, boff = 64 * s * RB;
+    // transposed fragment reads: per-tile offsets inside this wave's region
+    int TA[TI], TB[TJ];
+#pragma unroll
+    for (int i = 0; i < TI; ++i) TA[i] = tr_offset<true>(lane, 32 * i);
+#pragma unroll
+    for (int j = 0; j < TJ; ++j) TB[j] = tr_offset<false>(lane, 64 * (s & 1) + 32 * j);
+    const int aregion = 16384 * grp, bregion = 16384 * (s >> 1);
+    auto tr_frag = [&](const char* region, int off, int ks) {
+        const i16x4_t lo = read_tr16(region + off + 16 * ks * 256);
+        const i16x4_t hi = read_tr16(region + off + (16 * ks + 4) * 256);
+        const i32x2_t l2 = __builtin_bit_cast(i32x2_t, lo), h2 = __builtin_bit_cast(i32x2_t, hi);
+        return i32x4_t{l2[0], l2[1], h2[0], h2[1]};
+    };
 
     f32x16_t acc[TI][TJ];
 #pragma unroll
@@ -144,10 +232,10 @@ __device__ inline void gemm_tile(const TileArgs& a, int tm, int tn, int b, char*
     const int KT = (a.K + DU - 1) / DU;
 
     // prologue: K-tiles 0 and 1 in flight; K-tile 0 complete before the barrier
-    lb.issue(0, stB(0));
-    la.issue(0, stA(0));
-    lb.issue(1, stB(1));
-    la.issue(1, stA(1));
+    lb.issue(0, stB(0) + lhalf_b);
+    la.issue(0, stA(0) + lhalf_a);
+    lb.issue(1, stB(1) + lhalf_b);
+    la.issue(1, stA(1) + lhalf_a);
     wait_vmcnt<2 * NLD>();
     lds_barrier();
     if (grp == 1) barrier();  // group 1 runs one window behind group 0
@@ -157,25 +245,31 @@ __device__ inline void gemm_tile(const TileArgs& a, int tm, int tn, int b, char*
     for (int kt = 0; kt < KT; ++kt) {
         const char* sa = stA(kt) + aoff;
         const char* sb = stB(kt) + boff;
+        const char* ra = stA(kt) + aregion;
+        const char* rb = stB(kt) + bregion;
 #pragma unroll
         for (int half = 0; half < 2; ++half) {
             // ---- memory window
-            if (half == 0) lb.issue(kt + 2, stB(kt + 2));
+            if (half == 0) lb.issue(kt + 2, stB(kt + 2) + lhalf_b);
 #pragma unroll
             for (int k2 = 0; k2 < 2; ++k2) {
                 const int ks = 2 * half + k2;
 #pragma unroll
-                for (int i = 0; i < TI; ++i)
-                    fa[k2][i] = *(const i32x4_t*)(sa + P[ks ^ (2 * (i & 1))] + 4096 * i);
+                for (int i = 0; i < TI; ++i) {
+                    if constexpr (AK) fa[k2][i] = *(const i32x4_t*)(sa + P[ks ^ (2 * (i & 1))] + 4096 * i);
+                    else fa[k2][i] = tr_frag(ra, TA[i], ks);
+                }
 #pragma unroll
-                for (int j = 0; j < TJ; ++j)
-                    fb[k2][j] = *(const i32x4_t*)(sb + P[ks ^ (2 * j)] + 4096 * j);
+                for (int j = 0; j < TJ; ++j) {
+                    if constexpr (BK) fb[k2][j] = *(const i32x4_t*)(sb + P[ks ^ (2 * j)] + 4096 * j);
+                    else fb[k2][j] = tr_frag(rb, TB[j], ks);
+                }
             }
             if (half == 1) wait_vmcnt<2 * NLD>();
             lds_barrier();
             __builtin_amdgcn_sched_barrier(0);
             // ---- compute window
-            if (half == 1) la.issue(kt + 2, stA(kt));  // this group's half of A is consumed
+            if (half == 1) la.issue(kt + 2, stA(kt) + lhalf_a);  // this group's half of A is consumed
             __builtin_amdgcn_s_setprio(1);
             if constexpr (MODE == kF8) {
 #pragma unroll
@@ -225,12 +319,12 @@ __device__ inline void gemm_tile(const TileArgs& a, int tm, int tn, int b, char*
     if (grp == 0) barrier();  // matches group 1's extra barrier
 }
 
-template <int MODE, int OUT>
+template <int MODE, bool AK, bool BK, int OUT>
 __global__ void __launch_bounds__(kThreads8, 1) gemm_kernel(TileArgs a) {
     __shared__ __attribute__((aligned(16))) char smem[LDS_BYTES];
     int tb, tm, tn;
     map_tile(blockIdx.x, gridDim.x, a, tb, tm, tn);
-    v4::gemm_tile<MODE, OUT>(a, tm, tn, tb, smem);
+    v4::gemm_tile<MODE, AK, BK, OUT>(a, tm, tn, tb, smem);
 }
 
 }  // namespace v4
