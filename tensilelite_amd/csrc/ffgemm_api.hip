@@ -197,11 +197,15 @@ int prepare(const ffgemm_problem* p, const void* A, const void* B, const void* C
     if (tile && tile->mt_m) {
         if (!tile_supported(tile->mt_m, tile->mt_n))
             return fail(FFGEMM_ERR, "unsupported macro tile %dx%d", tile->mt_m, tile->mt_n);
-        if ((tile->waves_m && tile->waves_m != 2) || (tile->waves_n && tile->waves_n != 2) ||
-            (tile->mfma_mn && tile->mfma_mn != 32) || (tile->lds_buffers && tile->lds_buffers != 2) ||
-            (tile->depth_u && tile->depth_u != mode_depth_u(mode)))
-            return fail(FFGEMM_ERR, "unsupported tile knobs (waves 2x2, mfma 32, 2 LDS buffers, depth_u %d)",
-                        mode_depth_u(mode));
+        // the other knobs follow from the engine that runs this tile: zero, or
+        // the engine's own value (what ffgemm_query_tile reports)
+        const EngineGeometry geo = engine_geometry(mode, ta, !tb, tile->mt_m, tile->mt_n);
+        if ((tile->waves_m && tile->waves_m != geo.waves_m) || (tile->waves_n && tile->waves_n != geo.waves_n) ||
+            (tile->mfma_mn && tile->mfma_mn != 32) || (tile->lds_buffers && tile->lds_buffers != geo.lds_buffers) ||
+            (tile->depth_u && tile->depth_u != geo.depth_u))
+            return fail(FFGEMM_ERR,
+                        "unsupported tile knobs for MT%dx%d (waves %dx%d, mfma 32, %d LDS stages, depth_u %d)",
+                        tile->mt_m, tile->mt_n, geo.waves_m, geo.waves_n, geo.lds_buffers, geo.depth_u);
         t = Tile{tile->mt_m, tile->mt_n};
         group_m = tile->group_m;
     }
@@ -304,8 +308,11 @@ int ffgemm_query_tile(const ffgemm_problem* p, ffgemm_tile* out) {
     if (r != FFGEMM_SUCCESS) return r;
     if (out) {
         memset(out, 0, sizeof(*out));
-        out->mt_m = L.tile.m; out->mt_n = L.tile.n; out->depth_u = mode_depth_u(L.mode);
-        out->waves_m = 2; out->waves_n = 2; out->mfma_mn = 32; out->group_m = L.args.group_m; out->lds_buffers = 2;
+        const EngineGeometry geo = engine_geometry(L.mode, L.layout >= FFGEMM_TN, (L.layout & 1) == 0, L.tile.m,
+                                                   L.tile.n);
+        out->mt_m = L.tile.m; out->mt_n = L.tile.n; out->depth_u = geo.depth_u;
+        out->waves_m = geo.waves_m; out->waves_n = geo.waves_n; out->mfma_mn = 32; out->group_m = L.args.group_m;
+        out->lds_buffers = geo.lds_buffers;
     }
     return FFGEMM_SUCCESS;
 }
@@ -536,8 +543,11 @@ int ffgemm_grouped_plan_info(ffgemm_grouped_plan plan, int64_t* total_tiles, int
     if (grid) *grid = plan->grid;
     if (tile) {
         memset(tile, 0, sizeof(*tile));
-        tile->mt_m = plan->tile.m; tile->mt_n = plan->tile.n; tile->waves_m = 2; tile->waves_n = 2;
-        tile->mfma_mn = 32; tile->lds_buffers = 2; tile->depth_u = mode_depth_u(plan->mode); tile->group_m = 1;
+        const EngineGeometry geo = engine_geometry(plan->mode, plan->layout >= FFGEMM_TN, (plan->layout & 1) == 0,
+                                                   plan->tile.m, plan->tile.n);
+        tile->mt_m = plan->tile.m; tile->mt_n = plan->tile.n; tile->waves_m = geo.waves_m;
+        tile->waves_n = geo.waves_n; tile->mfma_mn = 32; tile->lds_buffers = geo.lds_buffers;
+        tile->depth_u = geo.depth_u; tile->group_m = 1;
     }
     return FFGEMM_SUCCESS;
 }

@@ -10,21 +10,56 @@ namespace ffgemm {
 // Supported macro tiles (MT_M x MT_N), 4 waves each.
 #define FFGEMM_TILES(X) X(256, 256) X(256, 128) X(128, 256) X(128, 128) X(64, 64)
 
+// Which engine runs a (mode, operand contiguity, tile), by what measured
+// fastest on MI355X at 4096^3: the eight-wave ping-pong engine (v4::) for
+// 256 x 256 TN and NT (M/N-contiguous operands through transposing LDS reads)
+// and fp8 TN; the split-ring engine (v3::) for 256 x 256 NN; the pipelined
+// 4-wave engine (v2::) for 256 x 256 TT and the other 16-bit tiles of 128 and
+// up; the first engine (gemm_tile.hpp) for the rest and for fp32 / bf16x3.
+enum Engine : int { kEngV1 = 1, kEngV2 = 2, kEngV3 = 3, kEngV4 = 4 };
+__host__ __device__ constexpr Engine engine_for(int mode, bool ak, bool bk, int tm, int tn) {
+    const bool k16 = (mode == kBF16 || mode == kF16);
+    if (tm == 256 && tn == 256 && ((k16 && ak == bk) || (mode == kF8 && ak && bk))) return kEngV4;
+    if (tm == 256 && tn == 256 && k16 && !ak && bk) return kEngV3;
+    if (k16 && tm >= 128 && tn >= 128) return kEngV2;
+    return kEngV1;
+}
+
+// Geometry of an engine, as ffgemm_query_tile reports it: waves along M and
+// N, K per main-loop stage, LDS stages of the deeper operand ring.
+struct EngineGeometry {
+    int waves_m, waves_n, depth_u, lds_buffers;
+};
+inline EngineGeometry engine_geometry(int mode, bool ak, bool bk, int tm, int tn) {
+    const int es = mode_elem_bytes(mode);
+    switch (engine_for(mode, ak, bk, tm, tn)) {
+        case kEngV4: return {2, 4, 128 / es, v4::NSB};
+        case kEngV3: return {2, 2, v3::DU, v3::NSB};
+        case kEngV2: {
+            int ns = 0, du = 0;
+#define FFGEMM_V2G(TM_, TN_, AK_, BK_) \
+    if (tm == TM_ && tn == TN_ && ak == AK_ && bk == BK_) { ns = v2::Cfg<TM_, TN_, AK_, BK_>::NS; du = v2::Cfg<TM_, TN_, AK_, BK_>::DU; }
+#define FFGEMM_V2L(TM_, TN_) FFGEMM_V2G(TM_, TN_, false, false) FFGEMM_V2G(TM_, TN_, false, true) \
+    FFGEMM_V2G(TM_, TN_, true, false) FFGEMM_V2G(TM_, TN_, true, true)
+            FFGEMM_V2L(256, 256) FFGEMM_V2L(256, 128) FFGEMM_V2L(128, 256) FFGEMM_V2L(128, 128)
+#undef FFGEMM_V2L
+#undef FFGEMM_V2G
+            return {2, 2, du, ns};
+        }
+        default: return {2, 2, mode_depth_u(mode), 2};
+    }
+}
+
 template <int MODE, bool AK, bool BK, int OUT>
 inline int launch_layout(int mt_m, int mt_n, const TileArgs& a, int grid, hipStream_t s) {
-    // 256 x 256, by what measured fastest on MI355X at 4096^3: the eight-wave
-    // ping-pong engine (v4::) for TN and NT (M/N-contiguous operands through
-    // transposing LDS reads) and fp8; the split-ring engine (v3::) for NN; the
-    // pipelined 4-wave engine (v2::) for TT and for the other tiles of 128 and
-    // up; the first engine for the rest (and fp32 / bf16x3 / fp8 below 256).
-    constexpr bool k16 = (MODE == kBF16 || MODE == kF16);
 #define FFGEMM_CASE(TM_, TN_)                                                                  \
     if (mt_m == TM_ && mt_n == TN_) {                                                          \
-        if constexpr (TM_ == 256 && TN_ == 256 && ((k16 && AK == BK) || (MODE == kF8 && AK && BK))) \
+        constexpr Engine E = engine_for(MODE, AK, BK, TM_, TN_);                               \
+        if constexpr (E == kEngV4)                                                             \
             hipLaunchKernelGGL((v4::gemm_kernel<MODE, AK, BK, OUT>), dim3(grid), dim3(v4::kThreads8), 0, s, a); \
-        else if constexpr (k16 && TM_ == 256 && TN_ == 256 && !AK && BK)                       \
+        else if constexpr (E == kEngV3)                                                        \
             hipLaunchKernelGGL((v3::gemm_kernel<MODE, AK, BK, OUT>), dim3(grid), dim3(kThreads), 0, s, a); \
-        else if constexpr (k16 && TM_ >= 128 && TN_ >= 128)                                    \
+        else if constexpr (E == kEngV2)                                                        \
             hipLaunchKernelGGL((v2::gemm_kernel<MODE, AK, BK, TM_, TN_, OUT>), dim3(grid), dim3(kThreads), 0, s, a); \
         else                                                                                   \
             hipLaunchKernelGGL((gemm_kernel<MODE, AK, BK, TM_, TN_, OUT>), dim3(grid), dim3(kThreads), 0, s, a); \

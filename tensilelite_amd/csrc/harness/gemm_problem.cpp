@@ -164,10 +164,11 @@ protected:
             snprintf(name, sizeof(name), "Cijk_Alik_Bljk_%s_MT%dx%dx%d_SE_K1", tns[g.dtype], tile_.mt_m, tile_.mt_n,
                      tile_.depth_u);
         else
-            snprintf(name, sizeof(name), "%s_col_%s_MT%dx%dx%d_4wave%s", pre[g.dtype], lay[g.layout], tile_.mt_m,
-                     tile_.mt_n, tile_.depth_u, g.compute ? "_bf16x3" : "");
-        FF_LOG("MT = [%d, %d], depth_u = %d, waves = [2, 2], mfma = %dx%d", tile_.mt_m, tile_.mt_n, tile_.depth_u,
-               tile_.mfma_mn, tile_.mfma_mn);
+            snprintf(name, sizeof(name), "%s_col_%s_MT%dx%dx%d_%dwave%s", pre[g.dtype], lay[g.layout], tile_.mt_m,
+                     tile_.mt_n, tile_.depth_u, tile_.waves_m * tile_.waves_n, g.compute ? "_bf16x3" : "");
+        FF_LOG("MT = [%d, %d], depth_u = %d, waves = [%d, %d], lds stages = %d, mfma = %dx%d", tile_.mt_m,
+               tile_.mt_n, tile_.depth_u, tile_.waves_m, tile_.waves_n, tile_.lds_buffers, tile_.mfma_mn,
+               tile_.mfma_mn);
         FF_LOG("kernel name: %s", name);
         score_.Calculation = 2.0 * g.M * g.N * g.K * g.batch;
         const DeviceInfo& dev = RuntimeHip::GetInstance()->Device();

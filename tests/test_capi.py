@@ -49,8 +49,7 @@ def _p(layout="NN", dtype=C.BF16, M=512, N=512, K=512, **kw):
 def test_query_tile_accepts_supported(layout, dtype):
     t = tl.query_tile(_p(layout, dtype, 4096, 4096, 4096))
     assert (t.mt_m, t.mt_n) == (256, 256)
-    assert t.waves_m == 2 and t.waves_n == 2 and t.mfma_mn == 32
-    assert t.depth_u == (64 if dtype != C.F32 else 32)
+    assert t.mfma_mn == 32
 
 
 def test_query_tile_small_shapes_pick_small_tiles():
@@ -110,3 +109,32 @@ def test_workspace_size_bf16x3_planes():
     assert tl.workspace_size(q) == 0
     r = tl.make_problem("TN", C.BF16, M, N, K, K, K, M, M)
     assert tl.workspace_size(r) == 0
+
+
+@pytest.mark.parametrize("layout,dtype,geo", [
+    ("TN", C.BF16, (256, 256, 2, 4, 64, 3)),    # eight-wave engine, K-contiguous rows
+    ("NT", C.F16, (256, 256, 2, 4, 64, 3)),     # eight-wave engine, transposing reads
+    ("NN", C.BF16, (256, 256, 2, 2, 64, 3)),    # split-ring engine
+    ("TT", C.BF16, (256, 256, 2, 2, 32, 4)),    # pipelined 4-wave engine
+    ("TN", C.F8E4M3, (256, 256, 2, 4, 128, 3)),
+    ("NN", C.F32, (256, 256, 2, 2, 32, 2)),
+])
+def test_query_tile_reports_engine_geometry(layout, dtype, geo):
+    """The tile record describes the engine that will run the 4096^3 problem."""
+    p = _p(layout, dtype, 4096, 4096, 4096)
+    if dtype == C.F8E4M3:
+        p.dtype_out = C.F32
+    t = tl.query_tile(p)
+    assert (t.mt_m, t.mt_n, t.waves_m, t.waves_n, t.depth_u, t.lds_buffers) == geo
+
+
+@pytest.mark.parametrize("layout", ["NN", "NT", "TN", "TT"])
+@pytest.mark.parametrize("dtype", [C.BF16, C.F32])
+@pytest.mark.parametrize("size", [4096, 1000, 296])
+def test_query_tile_record_round_trips(layout, dtype, size):
+    """A tile record reported by ffgemm_query_tile is accepted when passed back."""
+    p = _p(layout, dtype, size, size, size)
+    t = tl.query_tile(p)
+    t2 = tl.query_tile(p, t)
+    assert (t2.mt_m, t2.mt_n, t2.waves_m, t2.waves_n, t2.depth_u, t2.lds_buffers) == \
+        (t.mt_m, t.mt_n, t.waves_m, t.waves_n, t.depth_u, t.lds_buffers)
