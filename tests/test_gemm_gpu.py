@@ -437,3 +437,41 @@ def test_fp32_forms_8192_sampled_columns(cuda, compute):
     err = rel_fro(got, ref)
     _say(f"rel_fro={err:.3e}")
     assert err <= 1e-5
+
+
+@pytest.mark.parametrize("layout", LAYOUTS)
+@pytest.mark.parametrize("K", [8, 40, 64, 72, 136])
+def test_256_tile_short_k(cuda, layout, K):
+    """One or two K-tiles (partial ones included) on every 256 x 256 engine:
+    the pipelines' prologue, tail and barrier balance with almost no main loop."""
+    M, N = 264, 520
+    _say(f"case short K={K} {layout}")
+    rng = np.random.default_rng(K)
+    (ra, ca), (rb, cb) = op_dims(layout, M, N, K)
+    A = colmajor(rng, ra, ca, kind="normal")
+    B = colmajor(rng, rb, cb, kind="normal")
+    Ab, Bb = quantize_bits(A, "bf16"), quantize_bits(B, "bf16")
+    D = tl.gemm(layout, M, N, K, _dev16(Ab, "bf16", cuda), _dev16(Bb, "bf16", cuda), tile=(256, 256))
+    torch.cuda.synchronize()
+    ref = _f64_ref(layout, M, N, K, bits_to_f32(Ab, "bf16"), ra, bits_to_f32(Bb, "bf16"), rb)
+    bad, worst = elem_ok(D.float().cpu().numpy(), ref)
+    assert bad == 0, worst
+
+
+@pytest.mark.parametrize("K", [16, 112, 144])
+def test_fp8_256_tile_short_k(cuda, K):
+    M, N = 264, 300
+    _say(f"case fp8 short K={K}")
+    rng = np.random.default_rng(K)
+    A = rng.standard_normal(M * K).astype(np.float32)
+    B = rng.standard_normal(N * K).astype(np.float32)
+    sa, sb = np.float32(oracle.fp8_scale(np.abs(A).max())), np.float32(oracle.fp8_scale(np.abs(B).max()))
+    qa, qb = oracle.f32_to_e4m3_bits(A, sa), oracle.f32_to_e4m3_bits(B, sb)
+    D = tl.gemm("TN", M, N, K, torch.from_numpy(qa).to(cuda), torch.from_numpy(qb).to(cuda), lda=K, ldb=K,
+                scaleA=torch.tensor([sa], device=cuda), scaleB=torch.tensor([sb], device=cuda),
+                dtype_out=torch.float32, tile=(256, 256))
+    torch.cuda.synchronize()
+    deq_a = oracle.e4m3_bits_to_f32(qa).astype(np.float64) / sa
+    deq_b = oracle.e4m3_bits_to_f32(qb).astype(np.float64) / sb
+    ref = oracle.gemm_f64("TN", M, N, K, 1.0, deq_a, K, deq_b, K)
+    assert rel_fro(D.cpu().numpy(), ref) <= 5e-5
