@@ -339,8 +339,10 @@ bool x3_planes_apply(const ffgemm_problem* p) {
     if (p->M <= 0 || p->N <= 0 || p->K <= 0) return false;
     const int64_t batch = p->batch > 0 ? p->batch : 1;
     const int64_t row = 3 * x3_kp(p->K) * 2;  // bytes per plane row
-    // plane extents stay within 31-bit buffer offsets; split grid limits
-    return p->M * row < (1ll << 31) && p->N * row < (1ll << 31) && cdiv(std::max(p->M, p->N), 64) <= 65535 &&
+    // plane extents pass the bf16 GEMM's own 31-bit offset check (which allows
+    // 256 rows of tile overhang); split grid limits
+    return (p->M + 256) * row < (1ll << 31) && (p->N + 256) * row < (1ll << 31) &&
+           cdiv(std::max(p->M, p->N), 64) <= 65535 &&
            batch <= 65535;
 }
 int64_t x3_ws_bytes(const ffgemm_problem* p) {

@@ -475,3 +475,26 @@ def test_fp8_256_tile_short_k(cuda, K):
     deq_b = oracle.e4m3_bits_to_f32(qb).astype(np.float64) / sb
     ref = oracle.gemm_f64("TN", M, N, K, 1.0, deq_a, K, deq_b, K)
     assert rel_fro(D.cpu().numpy(), ref) <= 5e-5
+
+
+@pytest.mark.parametrize("layout", LAYOUTS)
+def test_batched_256_tile(cuda, layout):
+    """Batch strides with padding between batches on every 256 x 256 engine."""
+    M, N, K, batch = 300, 264, 136, 3
+    _say(f"case batched 256 {layout}")
+    rng = np.random.default_rng(9)
+    (ra, ca), (rb, cb) = op_dims(layout, M, N, K)
+    lda, ldb, ldd = (ra + 7) // 8 * 8, (rb + 7) // 8 * 8, (M + 7) // 8 * 8 + 8
+    bsA, bsB, bsD = lda * ca + 64, ldb * cb + 128, ldd * N + 32
+    A = np.concatenate([np.pad(colmajor(rng, ra, ca, lda, kind="normal"), (0, 64)) for _ in range(batch)])
+    B = np.concatenate([np.pad(colmajor(rng, rb, cb, ldb, kind="normal"), (0, 128)) for _ in range(batch)])
+    Ab, Bb = quantize_bits(A, "bf16"), quantize_bits(B, "bf16")
+    D = tl.gemm(layout, M, N, K, _dev16(Ab, "bf16", cuda), _dev16(Bb, "bf16", cuda), batch=batch, bsA=bsA,
+                bsB=bsB, bsD=bsD, lda=lda, ldb=ldb, ldd=ldd,
+                D=torch.zeros(bsD * batch, dtype=torch.bfloat16, device=cuda), tile=(256, 256))
+    torch.cuda.synchronize()
+    ref = _f64_ref(layout, M, N, K, bits_to_f32(Ab, "bf16"), lda, bits_to_f32(Bb, "bf16"), ldb, ldd=ldd,
+                   batch=batch, bsA=bsA, bsB=bsB, bsD=bsD)
+    g = extract(D.float().cpu().numpy(), M, N, ldd, batch, bsD)
+    r = extract(ref, M, N, ldd, batch, bsD)
+    assert elem_ok(g, r)[0] == 0
