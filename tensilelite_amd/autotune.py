@@ -4,10 +4,11 @@
 The reference sweeps padding, LDS buffers, MFMA size, MFMA tiles per wave
 (r, s), waves per group (x, y) and DepthU, keeps the best TFLOP/s and prints
 ``Performance = X TFLOPs``.  Here the knobs that exist in the gfx950 kernels
-are the macro tile (MT_M x MT_N, which fixes r = MT_M/64 and s = MT_N/64 MFMA
-tiles per wave on a 2 x 2 wave grid) and the tile-raster group height; DepthU
-is fixed at 128 bytes of K.  Results are kept in a JSON database keyed by
-(dtype, layout, M, N, K) and looked up by ``best_tile``.
+are the macro tile (MT_M x MT_N), the wave tiling (for 256 x 256: eight waves
+of 128 x 64, or four of 128 x 128 -- which also fixes the engine, its LDS
+ring depth and DepthU) and the tile-raster group height.  Results are kept in
+a JSON database keyed by (dtype, layout, M, N, K) and looked up by
+``best_tile``.
 """
 from __future__ import annotations
 
@@ -19,7 +20,8 @@ import torch
 from . import _capi as C
 from .gemm import make_problem, make_tile, run
 
-TILES = [(256, 256), (256, 128), (128, 256), (128, 128), (64, 64)]
+# (mt_m, mt_n, waves_m, waves_n); waves (0, 0) is the tile's default engine
+TILES = [(256, 256, 2, 4), (256, 256, 2, 2), (256, 128, 0, 0), (128, 256, 0, 0), (128, 128, 0, 0), (64, 64, 0, 0)]
 DB_PATH = os.environ.get("FFGEMM_TUNE_DB", os.path.join(os.path.dirname(os.path.abspath(__file__)), "tune_db.json"))
 
 
@@ -60,16 +62,17 @@ def tune_gemm(problem: C.Problem, A, B, Cm, D, tiles=TILES, group_ms=(0,), steps
     results = []
     flop = 2.0 * problem.M * problem.N * problem.K * max(1, problem.batch)
     for t in tiles:
+        wm, wn = (t[2], t[3]) if len(t) > 3 else (0, 0)
         for gm in group_ms:
-            tile = make_tile((t[0], t[1], gm))
+            tile = make_tile((t[0], t[1], gm, wm, wn))
             try:
                 ms = _time(lambda: run(problem, A, B, Cm, D, tile))
             except C.FFGemmError:
-                continue
+                continue  # no kernel for this tile / wave tiling with this dtype and layout
             tf = flop / (ms * 1e-3) / 1e12
-            results.append({"tile": [t[0], t[1], gm], "ms": ms, "tflops": tf})
+            results.append({"tile": [t[0], t[1], gm, wm, wn], "ms": ms, "tflops": tf})
             if verbose:
-                print(f"MT{t[0]}x{t[1]} gm={gm}: Performance = {tf:.2f} TFLOPs")
+                print(f"MT{t[0]}x{t[1]} waves={wm}x{wn} gm={gm}: Performance = {tf:.2f} TFLOPs")
     results.sort(key=lambda r: r["ms"])
     return {"best": tuple(results[0]["tile"]) if results else None, "results": results}
 

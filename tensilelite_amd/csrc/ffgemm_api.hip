@@ -122,6 +122,7 @@ struct Launch {
     int out;
     int layout;
     Tile tile;
+    int waves;  // 0: the default engine for the tile; 4 or 8: that wave count
     TileArgs args;
     int64_t grid;
 };
@@ -166,7 +167,7 @@ int prepare(const ffgemm_problem* p, const void* A, const void* B, const void* C
     if (p->lda >= lim || p->ldb >= lim || p->ldc >= lim || p->ldd >= lim) return fail(FFGEMM_ERR, "ld exceeds 2^31");
 
     if (M == 0 || N == 0) {  // nothing to compute or store
-        L.mode = mode; L.out = out; L.layout = p->layout; L.tile = kTiles[4]; L.grid = 0;
+        L.mode = mode; L.out = out; L.layout = p->layout; L.tile = kTiles[4]; L.grid = 0; L.waves = 0;
         memset(&L.args, 0, sizeof(L.args));
         return FFGEMM_SUCCESS;
     }
@@ -194,14 +195,23 @@ int prepare(const ffgemm_problem* p, const void* A, const void* B, const void* C
 
     Tile t = pick_tile(M, N, batch);
     int group_m = 0;
+    int waves = 0;
     if (tile && tile->mt_m) {
         if (!tile_supported(tile->mt_m, tile->mt_n))
             return fail(FFGEMM_ERR, "unsupported macro tile %dx%d", tile->mt_m, tile->mt_n);
-        // the other knobs follow from the engine that runs this tile: zero, or
-        // the engine's own value (what ffgemm_query_tile reports)
-        const EngineGeometry geo = engine_geometry(mode, ta, !tb, tile->mt_m, tile->mt_n);
-        if ((tile->waves_m && tile->waves_m != geo.waves_m) || (tile->waves_n && tile->waves_n != geo.waves_n) ||
-            (tile->mfma_mn && tile->mfma_mn != 32) || (tile->lds_buffers && tile->lds_buffers != geo.lds_buffers) ||
+        // waves_m x waves_n picks the wave tiling (0: the tile's default engine;
+        // 2 x 4: the eight-wave engine; 2 x 2: a four-wave engine); the other
+        // knobs follow from that engine: zero, or the engine's own value (what
+        // ffgemm_query_tile reports)
+        if (tile->waves_m || tile->waves_n) {
+            waves = tile->waves_m * tile->waves_n;
+            if (tile->waves_m != 2 || (waves != 4 && waves != 8) ||
+                engine_for(mode, ta, !tb, tile->mt_m, tile->mt_n, waves) == kEngNone)
+                return fail(FFGEMM_ERR, "no %dx%d-wave kernel for MT%dx%d with this dtype and layout",
+                            tile->waves_m, tile->waves_n, tile->mt_m, tile->mt_n);
+        }
+        const EngineGeometry geo = engine_geometry(mode, ta, !tb, tile->mt_m, tile->mt_n, waves);
+        if ((tile->mfma_mn && tile->mfma_mn != 32) || (tile->lds_buffers && tile->lds_buffers != geo.lds_buffers) ||
             (tile->depth_u && tile->depth_u != geo.depth_u))
             return fail(FFGEMM_ERR,
                         "unsupported tile knobs for MT%dx%d (waves %dx%d, mfma 32, %d LDS stages, depth_u %d)",
@@ -224,18 +234,18 @@ int prepare(const ffgemm_problem* p, const void* A, const void* B, const void* C
     a.tiles_m = (int32_t)tiles_m; a.tiles_n = (int32_t)tiles_n; a.batch = (int32_t)batch;
     a.group_m = group_m > 0 ? group_m : default_group_m(tiles_m, tiles_n);
     a.flags = (use_c ? kFlagBeta : 0u) | (p->round == FFGEMM_ROUND_TRUNCATE ? kFlagTrunc : 0u);
-    L.mode = mode; L.out = out; L.layout = p->layout; L.tile = t; L.grid = grid;
+    L.mode = mode; L.out = out; L.layout = p->layout; L.tile = t; L.grid = grid; L.waves = waves;
     return FFGEMM_SUCCESS;
 }
 
 int launch(const Launch& L, hipStream_t s) {
     int r = -1;
     switch (L.mode) {
-        case kBF16: r = launch_bf16(L.layout, L.out, L.tile.m, L.tile.n, L.args, (int)L.grid, s); break;
-        case kF16: r = launch_f16(L.layout, L.out, L.tile.m, L.tile.n, L.args, (int)L.grid, s); break;
-        case kF32: r = launch_f32(L.layout, L.out, L.tile.m, L.tile.n, L.args, (int)L.grid, s); break;
-        case kX3: r = launch_x3(L.layout, L.out, L.tile.m, L.tile.n, L.args, (int)L.grid, s); break;
-        case kF8: r = launch_f8(L.layout, L.out, L.tile.m, L.tile.n, L.args, (int)L.grid, s); break;
+        case kBF16: r = launch_bf16(L.layout, L.out, L.tile.m, L.tile.n, L.waves, L.args, (int)L.grid, s); break;
+        case kF16: r = launch_f16(L.layout, L.out, L.tile.m, L.tile.n, L.waves, L.args, (int)L.grid, s); break;
+        case kF32: r = launch_f32(L.layout, L.out, L.tile.m, L.tile.n, L.waves, L.args, (int)L.grid, s); break;
+        case kX3: r = launch_x3(L.layout, L.out, L.tile.m, L.tile.n, L.waves, L.args, (int)L.grid, s); break;
+        case kF8: r = launch_f8(L.layout, L.out, L.tile.m, L.tile.n, L.waves, L.args, (int)L.grid, s); break;
     }
     if (r != 0) return fail(FFGEMM_ERR, "no kernel for mode %d out %d layout %d tile %dx%d", L.mode, L.out,
                             L.layout, L.tile.m, L.tile.n);
@@ -309,7 +319,7 @@ int ffgemm_query_tile(const ffgemm_problem* p, ffgemm_tile* out) {
     if (out) {
         memset(out, 0, sizeof(*out));
         const EngineGeometry geo = engine_geometry(L.mode, L.layout >= FFGEMM_TN, (L.layout & 1) == 0, L.tile.m,
-                                                   L.tile.n);
+                                                   L.tile.n, L.waves);
         out->mt_m = L.tile.m; out->mt_n = L.tile.n; out->depth_u = geo.depth_u;
         out->waves_m = geo.waves_m; out->waves_n = geo.waves_n; out->mfma_mn = 32; out->group_m = L.args.group_m;
         out->lds_buffers = geo.lds_buffers;

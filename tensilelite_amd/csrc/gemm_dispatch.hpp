@@ -7,22 +7,29 @@
 
 namespace ffgemm {
 
-// Supported macro tiles (MT_M x MT_N), 4 waves each.
+// Supported macro tiles (MT_M x MT_N).
 #define FFGEMM_TILES(X) X(256, 256) X(256, 128) X(128, 256) X(128, 128) X(64, 64)
 
-// Which engine runs a (mode, operand contiguity, tile), by what measured
-// fastest on MI355X at 4096^3: the eight-wave ping-pong engine (v4::) for
-// 256 x 256 TN and NT (M/N-contiguous operands through transposing LDS reads)
-// and fp8 TN; the split-ring engine (v3::) for 256 x 256 NN; the pipelined
-// 4-wave engine (v2::) for 256 x 256 TT and the other 16-bit tiles of 128 and
-// up; the first engine (gemm_tile.hpp) for the rest and for fp32 / bf16x3.
-enum Engine : int { kEngV1 = 1, kEngV2 = 2, kEngV3 = 3, kEngV4 = 4 };
-__host__ __device__ constexpr Engine engine_for(int mode, bool ak, bool bk, int tm, int tn) {
+// Which engine runs a (mode, operand contiguity, tile, waves).  waves = 0 takes
+// the default, what measured fastest on MI355X at 4096^3: the eight-wave
+// ping-pong engine (v4::) for 256 x 256 TN and NT (M/N-contiguous operands
+// through transposing LDS reads) and fp8 TN; the split-ring engine (v3::) for
+// 256 x 256 NN; the pipelined 4-wave engine (v2::) for 256 x 256 TT and the
+// other 16-bit tiles of 128 and up; the first engine (gemm_tile.hpp) for the
+// rest and for fp32 / bf16x3.  waves = 8 or 4 asks for the eight-wave or a
+// four-wave engine (the wave-tiling knob of the tile record, which the
+// autotuner sweeps); kEngNone when the combination has no kernel.
+enum Engine : int { kEngNone = 0, kEngV1 = 1, kEngV2 = 2, kEngV3 = 3, kEngV4 = 4 };
+__host__ __device__ constexpr Engine engine_for(int mode, bool ak, bool bk, int tm, int tn, int waves = 0) {
     const bool k16 = (mode == kBF16 || mode == kF16);
-    if (tm == 256 && tn == 256 && ((k16 && ak == bk) || (mode == kF8 && ak && bk))) return kEngV4;
-    if (tm == 256 && tn == 256 && k16 && !ak && bk) return kEngV3;
-    if (k16 && tm >= 128 && tn >= 128) return kEngV2;
-    return kEngV1;
+    const bool big = (tm == 256 && tn == 256);
+    const bool v4ok = big && (k16 || (mode == kF8 && ak && bk));
+    const Engine four = (big && k16 && !ak && bk) ? kEngV3 : ((k16 && tm >= 128 && tn >= 128) ? kEngV2 : kEngV1);
+    if (waves == 8) return v4ok ? kEngV4 : kEngNone;
+    if (waves == 4) return four;
+    if (waves != 0) return kEngNone;
+    if (big && ((k16 && ak == bk) || (mode == kF8 && ak && bk))) return kEngV4;
+    return four;
 }
 
 // Geometry of an engine, as ffgemm_query_tile reports it: waves along M and
@@ -30,9 +37,9 @@ __host__ __device__ constexpr Engine engine_for(int mode, bool ak, bool bk, int 
 struct EngineGeometry {
     int waves_m, waves_n, depth_u, lds_buffers;
 };
-inline EngineGeometry engine_geometry(int mode, bool ak, bool bk, int tm, int tn) {
+inline EngineGeometry engine_geometry(int mode, bool ak, bool bk, int tm, int tn, int waves = 0) {
     const int es = mode_elem_bytes(mode);
-    switch (engine_for(mode, ak, bk, tm, tn)) {
+    switch (engine_for(mode, ak, bk, tm, tn, waves)) {
         case kEngV4: return {2, 4, 128 / es, v4::NSB};
         case kEngV3: return {2, 2, v3::DU, v3::NSB};
         case kEngV2: {
@@ -46,24 +53,37 @@ inline EngineGeometry engine_geometry(int mode, bool ak, bool bk, int tm, int tn
 #undef FFGEMM_V2G
             return {2, 2, du, ns};
         }
-        default: return {2, 2, mode_depth_u(mode), 2};
+        case kEngV1: return {2, 2, mode_depth_u(mode), 2};
+        default: return {0, 0, 0, 0};
     }
 }
 
+template <Engine E, int MODE, bool AK, bool BK, int TM, int TN, int OUT>
+inline void launch_engine(const TileArgs& a, int grid, hipStream_t s) {
+    if constexpr (E == kEngV4)
+        hipLaunchKernelGGL((v4::gemm_kernel<MODE, AK, BK, OUT>), dim3(grid), dim3(v4::kThreads8), 0, s, a);
+    else if constexpr (E == kEngV3)
+        hipLaunchKernelGGL((v3::gemm_kernel<MODE, AK, BK, OUT>), dim3(grid), dim3(kThreads), 0, s, a);
+    else if constexpr (E == kEngV2)
+        hipLaunchKernelGGL((v2::gemm_kernel<MODE, AK, BK, TM, TN, OUT>), dim3(grid), dim3(kThreads), 0, s, a);
+    else if constexpr (E == kEngV1)
+        hipLaunchKernelGGL((gemm_kernel<MODE, AK, BK, TM, TN, OUT>), dim3(grid), dim3(kThreads), 0, s, a);
+}
+
+// Launches the engine engine_for picks; only the engines reachable for this
+// (mode, layout, tile) are instantiated.
 template <int MODE, bool AK, bool BK, int OUT>
-inline int launch_layout(int mt_m, int mt_n, const TileArgs& a, int grid, hipStream_t s) {
+inline int launch_layout(int mt_m, int mt_n, int waves, const TileArgs& a, int grid, hipStream_t s) {
 #define FFGEMM_CASE(TM_, TN_)                                                                  \
     if (mt_m == TM_ && mt_n == TN_) {                                                          \
-        constexpr Engine E = engine_for(MODE, AK, BK, TM_, TN_);                               \
-        if constexpr (E == kEngV4)                                                             \
-            hipLaunchKernelGGL((v4::gemm_kernel<MODE, AK, BK, OUT>), dim3(grid), dim3(v4::kThreads8), 0, s, a); \
-        else if constexpr (E == kEngV3)                                                        \
-            hipLaunchKernelGGL((v3::gemm_kernel<MODE, AK, BK, OUT>), dim3(grid), dim3(kThreads), 0, s, a); \
-        else if constexpr (E == kEngV2)                                                        \
-            hipLaunchKernelGGL((v2::gemm_kernel<MODE, AK, BK, TM_, TN_, OUT>), dim3(grid), dim3(kThreads), 0, s, a); \
-        else                                                                                   \
-            hipLaunchKernelGGL((gemm_kernel<MODE, AK, BK, TM_, TN_, OUT>), dim3(grid), dim3(kThreads), 0, s, a); \
-        return 0;                                                                              \
+        constexpr Engine E0 = engine_for(MODE, AK, BK, TM_, TN_, 0);                           \
+        constexpr Engine E4 = engine_for(MODE, AK, BK, TM_, TN_, 4);                           \
+        constexpr Engine E8 = engine_for(MODE, AK, BK, TM_, TN_, 8);                           \
+        const Engine e = engine_for(MODE, AK, BK, TM_, TN_, waves);                            \
+        if (e == E0) { launch_engine<E0, MODE, AK, BK, TM_, TN_, OUT>(a, grid, s); return 0; } \
+        if (e == E4) { launch_engine<E4, MODE, AK, BK, TM_, TN_, OUT>(a, grid, s); return 0; } \
+        if (e == E8 && E8 != kEngNone) { launch_engine<E8, MODE, AK, BK, TM_, TN_, OUT>(a, grid, s); return 0; } \
+        return -1;                                                                             \
     }
     FFGEMM_TILES(FFGEMM_CASE)
 #undef FFGEMM_CASE
@@ -72,21 +92,22 @@ inline int launch_layout(int mt_m, int mt_n, const TileArgs& a, int grid, hipStr
 
 // layout: 0 NN, 1 NT, 2 TN, 3 TT.  A is K-contiguous for TN/TT, B for NN/TN.
 template <int MODE, int OUT>
-inline int launch_mode(int layout, int mt_m, int mt_n, const TileArgs& a, int grid, hipStream_t s) {
+inline int launch_mode(int layout, int mt_m, int mt_n, int waves, const TileArgs& a, int grid, hipStream_t s) {
     switch (layout) {
-        case 0: return launch_layout<MODE, false, true, OUT>(mt_m, mt_n, a, grid, s);
-        case 1: return launch_layout<MODE, false, false, OUT>(mt_m, mt_n, a, grid, s);
-        case 2: return launch_layout<MODE, true, true, OUT>(mt_m, mt_n, a, grid, s);
-        case 3: return launch_layout<MODE, true, false, OUT>(mt_m, mt_n, a, grid, s);
+        case 0: return launch_layout<MODE, false, true, OUT>(mt_m, mt_n, waves, a, grid, s);
+        case 1: return launch_layout<MODE, false, false, OUT>(mt_m, mt_n, waves, a, grid, s);
+        case 2: return launch_layout<MODE, true, true, OUT>(mt_m, mt_n, waves, a, grid, s);
+        case 3: return launch_layout<MODE, true, false, OUT>(mt_m, mt_n, waves, a, grid, s);
     }
     return -1;
 }
 
 // Per-mode entry points (defined in gemm_<mode>.hip).
-int launch_bf16(int layout, int out, int mt_m, int mt_n, const TileArgs& a, int grid, hipStream_t s);
-int launch_f16(int layout, int out, int mt_m, int mt_n, const TileArgs& a, int grid, hipStream_t s);
-int launch_f32(int layout, int out, int mt_m, int mt_n, const TileArgs& a, int grid, hipStream_t s);
-int launch_x3(int layout, int out, int mt_m, int mt_n, const TileArgs& a, int grid, hipStream_t s);
-int launch_f8(int layout, int out, int mt_m, int mt_n, const TileArgs& a, int grid, hipStream_t s);
+// waves: 0 (default engine), 4 or 8 (see engine_for)
+int launch_bf16(int layout, int out, int mt_m, int mt_n, int waves, const TileArgs& a, int grid, hipStream_t s);
+int launch_f16(int layout, int out, int mt_m, int mt_n, int waves, const TileArgs& a, int grid, hipStream_t s);
+int launch_f32(int layout, int out, int mt_m, int mt_n, int waves, const TileArgs& a, int grid, hipStream_t s);
+int launch_x3(int layout, int out, int mt_m, int mt_n, int waves, const TileArgs& a, int grid, hipStream_t s);
+int launch_f8(int layout, int out, int mt_m, int mt_n, int waves, const TileArgs& a, int grid, hipStream_t s);
 
 }  // namespace ffgemm

@@ -60,6 +60,10 @@ def make_problem(layout, dtype, M, N, K, lda, ldb, ldc, ldd, alpha=1.0, beta=0.0
 
 
 def make_tile(tile) -> C.Tile | None:
+    """Tile record from (mt_m, mt_n[, group_m[, waves_m, waves_n]]) or a Tile.
+
+    waves_m x waves_n picks the wave tiling: 2 x 4 the eight-wave engine,
+    2 x 2 a four-wave one, 0 x 0 (or absent) the tile's default."""
     if tile is None:
         return None
     if isinstance(tile, C.Tile):
@@ -68,6 +72,8 @@ def make_tile(tile) -> C.Tile | None:
     t.mt_m, t.mt_n = int(tile[0]), int(tile[1])
     if len(tile) > 2:
         t.group_m = int(tile[2])
+    if len(tile) > 4:
+        t.waves_m, t.waves_n = int(tile[3]), int(tile[4])
     return t
 
 

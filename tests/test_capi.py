@@ -138,3 +138,34 @@ def test_query_tile_record_round_trips(layout, dtype, size):
     t2 = tl.query_tile(p, t)
     assert (t2.mt_m, t2.mt_n, t2.waves_m, t2.waves_n, t2.depth_u, t2.lds_buffers) == \
         (t.mt_m, t.mt_n, t.waves_m, t.waves_n, t.depth_u, t.lds_buffers)
+
+
+@pytest.mark.parametrize("layout,dtype,waves,geo", [
+    ("TN", C.BF16, (2, 2), (2, 2, 32, 4)),     # four-wave pipelined engine
+    ("NN", C.BF16, (2, 4), (2, 4, 64, 3)),     # eight-wave engine, transposing reads for A
+    ("NN", C.BF16, (2, 2), (2, 2, 64, 3)),     # split-ring engine (the default)
+    ("TT", C.F16, (2, 4), (2, 4, 64, 3)),
+    ("TN", C.F8E4M3, (2, 2), (2, 2, 128, 2)),  # fp8 on the first engine
+])
+def test_wave_tiling_knob(layout, dtype, waves, geo):
+    p = _p(layout, dtype, 4096, 4096, 4096)
+    if dtype == C.F8E4M3:
+        p.dtype_out = C.F32
+    t = C.Tile()
+    t.mt_m, t.mt_n, t.waves_m, t.waves_n = 256, 256, waves[0], waves[1]
+    r = tl.query_tile(p, t)
+    assert (r.waves_m, r.waves_n, r.depth_u, r.lds_buffers) == geo
+
+
+@pytest.mark.parametrize("layout,dtype,tile,waves", [
+    ("NN", C.F32, (256, 256), (2, 4)),   # no eight-wave fp32 kernel
+    ("TN", C.BF16, (128, 128), (2, 4)),  # eight waves only at 256 x 256
+    ("TN", C.BF16, (256, 256), (4, 2)),
+    ("TN", C.BF16, (256, 256), (2, 3)),
+])
+def test_wave_tiling_knob_rejects(layout, dtype, tile, waves):
+    p = _p(layout, dtype, 4096, 4096, 4096)
+    t = C.Tile()
+    t.mt_m, t.mt_n, t.waves_m, t.waves_n = tile[0], tile[1], waves[0], waves[1]
+    with pytest.raises(C.FFGemmError, match="RTN_ERR"):
+        tl.query_tile(p, t)

@@ -498,3 +498,24 @@ def test_batched_256_tile(cuda, layout):
     g = extract(D.float().cpu().numpy(), M, N, ldd, batch, bsD)
     r = extract(ref, M, N, ldd, batch, bsD)
     assert elem_ok(g, r)[0] == 0
+
+
+@pytest.mark.parametrize("name", ["bf16", "f16"])
+@pytest.mark.parametrize("layout", LAYOUTS)
+@pytest.mark.parametrize("waves", [(2, 4), (2, 2)])
+def test_256_wave_tilings(cuda, name, layout, waves):
+    """Both wave tilings of the 256 x 256 tile on every layout (the eight-wave
+    engine with transposing reads for NN and TT, the four-wave engines for TN
+    and NT included), ragged in M, N and K."""
+    M, N, K = 328, 272, 200
+    _say(f"case waves {waves} {name} {layout}")
+    rng = np.random.default_rng(13)
+    (ra, ca), (rb, cb) = op_dims(layout, M, N, K)
+    A = colmajor(rng, ra, ca, kind="normal", scale=1 / np.sqrt(K))
+    B = colmajor(rng, rb, cb, kind="normal", scale=1 / np.sqrt(K))
+    Ab, Bb = quantize_bits(A, name), quantize_bits(B, name)
+    D = tl.gemm(layout, M, N, K, _dev16(Ab, name, cuda), _dev16(Bb, name, cuda), tile=(256, 256, 0) + waves)
+    torch.cuda.synchronize()
+    ref = _f64_ref(layout, M, N, K, bits_to_f32(Ab, name), ra, bits_to_f32(Bb, name), rb)
+    bad, worst = elem_ok(D.float().cpu().numpy(), ref)
+    assert bad == 0, worst
