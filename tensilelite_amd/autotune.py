@@ -39,14 +39,16 @@ def _time(fn, steps=10, warmup=3):
     return e0.elapsed_time(e1) / steps
 
 
-def load_db(path=DB_PATH):
+def load_db(path=None):
+    path = path or DB_PATH
     if os.path.isfile(path):
         with open(path) as f:
             return json.load(f)
     return {}
 
 
-def save_db(db, path=DB_PATH):
+def save_db(db, path=None):
+    path = path or DB_PATH
     tmp = path + ".tmp"
     with open(tmp, "w") as f:
         json.dump(db, f, indent=1, sort_keys=True)
@@ -77,7 +79,7 @@ def tune_gemm(problem: C.Problem, A, B, Cm, D, tiles=TILES, group_ms=(0,), steps
     return {"best": tuple(results[0]["tile"]) if results else None, "results": results}
 
 
-def tune_and_record(problem, A, B, Cm, D, db=None, path=DB_PATH, **kw):
+def tune_and_record(problem, A, B, Cm, D, db=None, path=None, **kw):
     db = load_db(path) if db is None else db
     r = tune_gemm(problem, A, B, Cm, D, **kw)
     if r["best"] is not None:
@@ -88,7 +90,7 @@ def tune_and_record(problem, A, B, Cm, D, db=None, path=DB_PATH, **kw):
     return r
 
 
-def best_tile(dtype_code, layout, M, N, K, path=DB_PATH):
+def best_tile(dtype_code, layout, M, N, K, path=None):
     e = load_db(path).get(key(dtype_code, layout, M, N, K))
     return tuple(e["tile"]) if e else None
 

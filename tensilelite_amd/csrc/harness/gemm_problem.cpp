@@ -143,6 +143,11 @@ protected:
             int z = (int)ca->Int("mfma-mn");
             tile_.mt_m = z * (int)ca->Int("mfma-pttn0") * (int)ca->Int("wave-pttn0");
             tile_.mt_n = z * (int)ca->Int("mfma-pttn1") * (int)ca->Int("wave-pttn1");
+            // -x -y are the waves per group: the wave tiling of the tile record
+            if (ca->Given("wave-pttn0") || ca->Given("wave-pttn1")) {
+                tile_.waves_m = (int)ca->Int("wave-pttn0");
+                tile_.waves_n = (int)ca->Int("wave-pttn1");
+            }
             explicit_tile_ = true;
         }
         int st = ffgemm_query_tile(&prob_, &tile_);

@@ -84,12 +84,23 @@ def workspace_size(problem: C.Problem) -> int:
     return n.value
 
 
+def _tuned(problem: C.Problem):
+    from .autotune import best_tile
+    return best_tile(problem.dtype, ["NN", "NT", "TN", "TT"][problem.layout], problem.M, problem.N, problem.K)
+
+
 def run(problem: C.Problem, A, B, Cm, D, tile=None, stream=None, workspace="auto") -> None:
     """ffgemm_run on device buffers (tensors or raw pointers).
 
+    tile: None (the library's choice), a tile record / tuple (see make_tile),
+    or "tuned" (the autotuner database's best for this problem, else None).
     workspace: "auto" allocates what ffgemm_workspace_size asks for (from
     torch's caching allocator) and runs ffgemm_run_ws; None runs ffgemm_run;
     a uint8 device tensor is used as given."""
+    if isinstance(tile, str):
+        if tile != "tuned":
+            raise ValueError(f"tile={tile!r}: expected None, a tile, or 'tuned'")
+        tile = _tuned(problem)
     t = make_tile(tile)
     tp = ctypes.byref(t) if t is not None else None
     if isinstance(workspace, str):

@@ -169,3 +169,17 @@ def test_wave_tiling_knob_rejects(layout, dtype, tile, waves):
     t.mt_m, t.mt_n, t.waves_m, t.waves_n = tile[0], tile[1], waves[0], waves[1]
     with pytest.raises(C.FFGemmError, match="RTN_ERR"):
         tl.query_tile(p, t)
+
+
+def test_tuned_tile_lookup(tmp_path, monkeypatch):
+    """tile='tuned' reads the autotuner database (a missing entry means the library's choice)."""
+    import importlib
+    from tensilelite_amd import autotune
+    G = importlib.import_module("tensilelite_amd.gemm")   # the module (the package re-exports gemm())
+    monkeypatch.setattr(autotune, "DB_PATH", str(tmp_path / "db.json"))
+    autotune.save_db({autotune.key(C.BF16, "TN", 4096, 4096, 4096): {"tile": [256, 256, 0, 2, 2], "tflops": 1.0}})
+    p = _p("TN", C.BF16, 4096, 4096, 4096)
+    assert list(G._tuned(p)) == [256, 256, 0, 2, 2]
+    t = tl.query_tile(p, G.make_tile(G._tuned(p)))
+    assert (t.waves_m, t.waves_n) == (2, 2)
+    assert G._tuned(_p("TN", C.BF16, 1024, 1024, 1024)) is None
