@@ -49,6 +49,9 @@ def _perf(stdout: str) -> float:
     ["-d", "3", "-m", "512", "-n", "512", "-k", "512", "--round", "truncate", "-r", "2", "-s", "2", "-x", "2", "-y", "2"],
     ["-d", "1", "-m", "512", "-n", "512", "-k", "512", "--compute", "bf16x3", "--init", "normal"],
     ["-d", "4", "-m", "512", "-n", "512", "-k", "512", "--init", "normal"],
+    # the reference's tile flags asking for 256 x 256 with 2 x 4 waves (the eight-wave engine)
+    ["-d", "3", "-m", "1024", "-n", "1024", "-k", "1024", "-r", "4", "-s", "2", "-x", "2", "-y", "4", "-z", "32",
+     "--init", "normal"],
 ])
 def test_cli_run_verifies_and_reports(cuda, args):
     _need_bin()
