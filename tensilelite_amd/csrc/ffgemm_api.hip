@@ -11,6 +11,7 @@
 #include <cstdio>
 #include <cstring>
 #include <mutex>
+#include <queue>
 #include <string>
 #include <vector>
 
@@ -458,7 +459,10 @@ int ffgemm_convert(const float* src, void* dst, int64_t n, int dtype_dst, void* 
 // Grouped GEMM plans
 // ---------------------------------------------------------------------------
 struct ffgemm_grouped_plan_s {
-    GroupDesc* d_descs = nullptr;
+    void* d_mem = nullptr;  // descriptors, then the schedule's first[grid + 1], then the tiles
+    const GroupDesc* d_descs = nullptr;
+    const int32_t* d_first = nullptr;
+    const GroupTile* d_sched = nullptr;
     int count = 0;
     int64_t total = 0;
     int grid = 0;
@@ -481,10 +485,8 @@ int ffgemm_grouped_plan_create(ffgemm_grouped_plan* plan, int count, const ffgem
         if (!tile_supported(tile->mt_m, tile->mt_n))
             return fail(FFGEMM_ERR, "unsupported macro tile %dx%d", tile->mt_m, tile->mt_n);
         t = Tile{tile->mt_m, tile->mt_n};
-        if (t.m == 256 && t.n == 256) return fail(FFGEMM_ERR, "grouped GEMM supports tiles up to 256x128");
     } else {
         for (const Tile& c : kTiles) {
-            if (c.m == 256 && c.n == 256) continue;  // no grouped kernel for this tile
             int64_t n = 0;
             for (int i = 0; i < count; ++i) n += cdiv(probs[i].M, c.m) * cdiv(probs[i].N, c.n);
             t = c;
@@ -495,7 +497,6 @@ int ffgemm_grouped_plan_create(ffgemm_grouped_plan* plan, int count, const ffgem
     memset(&tt, 0, sizeof(tt));
     tt.mt_m = t.m; tt.mt_n = t.n;
     std::vector<GroupDesc> descs(count);
-    std::vector<int> order(count);
     int mode = -1, out = -1, layout = -1;
     for (int i = 0; i < count; ++i) {
         const ffgemm_problem& p = probs[i];
@@ -508,31 +509,64 @@ int ffgemm_grouped_plan_create(ffgemm_grouped_plan* plan, int count, const ffgem
             return fail(FFGEMM_ERR, "grouped problems must share dtype, compute and layout");
         descs[i].a = L.args;
         descs[i].a.group_m = 1;
-        order[i] = i;
     }
-    // longest tiles (largest K) first, so short tiles fill the tail of the schedule
+    // Tile schedule, longest-processing-time first: tiles in K-descending
+    // order, each dealt to the workgroup with the least work so far (cost of a
+    // tile: its K-steps of 64 plus two for the prologue and epilogue), so the
+    // short tiles fill the tail of every workgroup's list.
+    std::vector<int> order(count);
+    for (int i = 0; i < count; ++i) order[i] = i;
     std::stable_sort(order.begin(), order.end(), [&](int x, int y) { return probs[x].K > probs[y].K; });
-    std::vector<GroupDesc> sorted;
     int64_t total = 0;
-    for (int i : order) {
-        GroupDesc d = descs[i];
-        int64_t nt = (int64_t)d.a.tiles_m * d.a.tiles_n;
-        if (nt == 0) continue;
-        d.tile_start = total;
-        total += nt;
-        sorted.push_back(d);
+    for (int i = 0; i < count; ++i) total += (int64_t)descs[i].a.tiles_m * descs[i].a.tiles_n;
+    if (total > INT32_MAX) return fail(FFGEMM_ERR, "grouped GEMM of %lld tiles exceeds 2^31", (long long)total);
+    const int grid = (int)std::max<int64_t>(1, std::min<int64_t>(total, (int64_t)cu_count() * tile_occupancy(t.m, t.n, layout)));
+    std::vector<std::vector<GroupTile>> lists(grid);
+    {
+        using Slot = std::pair<int64_t, int>;  // (work so far, workgroup); ties go to the lower workgroup
+        std::priority_queue<Slot, std::vector<Slot>, std::greater<Slot>> heap;
+        for (int b = 0; b < grid; ++b) heap.push({0, b});
+        for (int i : order) {
+            const int nt = descs[i].a.tiles_m * descs[i].a.tiles_n;
+            const int64_t cost = cdiv(probs[i].K, 64) + 2;
+            for (int k = 0; k < nt; ++k) {
+                Slot w = heap.top();
+                heap.pop();
+                lists[w.second].push_back(GroupTile{i, k});
+                heap.push({w.first + cost, w.second});
+            }
+        }
     }
+    std::vector<int32_t> first(grid + 1, 0);
+    std::vector<GroupTile> sched;
+    sched.reserve((size_t)total);
+    for (int b = 0; b < grid; ++b) {
+        first[b] = (int32_t)sched.size();
+        sched.insert(sched.end(), lists[b].begin(), lists[b].end());
+    }
+    first[grid] = (int32_t)sched.size();
+
     ffgemm_grouped_plan_s* P = new ffgemm_grouped_plan_s();
-    P->count = (int)sorted.size();
+    P->count = count;
     P->total = total;
-    P->grid = (int)std::max<int64_t>(1, std::min<int64_t>(total, (int64_t)cu_count() * tile_occupancy(t.m, t.n, layout)));
+    P->grid = grid;
     P->mode = mode; P->out = out; P->layout = layout; P->tile = t;
     hipGetDevice(&P->device);
-    if (P->count > 0) {
-        hipError_t e = hipMalloc((void**)&P->d_descs, sizeof(GroupDesc) * P->count);
-        if (e != hipSuccess) { delete P; return hip_fail(e, "hipMalloc(descriptors)"); }
-        e = hipMemcpy(P->d_descs, sorted.data(), sizeof(GroupDesc) * P->count, hipMemcpyHostToDevice);
-        if (e != hipSuccess) { hipFree(P->d_descs); delete P; return hip_fail(e, "hipMemcpy(descriptors)"); }
+    if (total > 0) {
+        const size_t off_first = round256(sizeof(GroupDesc) * count);
+        const size_t off_sched = off_first + round256(sizeof(int32_t) * (grid + 1));
+        const size_t bytes = off_sched + sizeof(GroupTile) * sched.size();
+        std::vector<char> host(bytes, 0);
+        memcpy(host.data(), descs.data(), sizeof(GroupDesc) * count);
+        memcpy(host.data() + off_first, first.data(), sizeof(int32_t) * (grid + 1));
+        memcpy(host.data() + off_sched, sched.data(), sizeof(GroupTile) * sched.size());
+        hipError_t e = hipMalloc(&P->d_mem, bytes);
+        if (e != hipSuccess) { delete P; return hip_fail(e, "hipMalloc(grouped plan)"); }
+        e = hipMemcpy(P->d_mem, host.data(), bytes, hipMemcpyHostToDevice);
+        if (e != hipSuccess) { hipFree(P->d_mem); delete P; return hip_fail(e, "hipMemcpy(grouped plan)"); }
+        P->d_descs = (const GroupDesc*)P->d_mem;
+        P->d_first = (const int32_t*)((const char*)P->d_mem + off_first);
+        P->d_sched = (const GroupTile*)((const char*)P->d_mem + off_sched);
     }
     *plan = P;
     return FFGEMM_SUCCESS;
@@ -542,7 +576,7 @@ int ffgemm_grouped_plan_run(ffgemm_grouped_plan plan, void* stream) {
     if (!plan) return fail(FFGEMM_ERR, "null plan");
     if (plan->total == 0) return FFGEMM_SUCCESS;
     int r = launch_grouped(plan->mode, plan->out, plan->layout, plan->tile.m, plan->tile.n, plan->d_descs,
-                           plan->count, plan->total, plan->grid, (hipStream_t)stream);
+                           plan->d_first, plan->d_sched, plan->grid, (hipStream_t)stream);
     if (r != 0) return fail(FFGEMM_ERR, "no grouped kernel for mode %d out %d layout %d", plan->mode, plan->out,
                             plan->layout);
     hipError_t e = hipGetLastError();
@@ -566,7 +600,7 @@ int ffgemm_grouped_plan_info(ffgemm_grouped_plan plan, int64_t* total_tiles, int
 
 int ffgemm_grouped_plan_destroy(ffgemm_grouped_plan plan) {
     if (!plan) return FFGEMM_SUCCESS;
-    if (plan->d_descs) hipFree(plan->d_descs);
+    if (plan->d_mem) hipFree(plan->d_mem);
     delete plan;
     return FFGEMM_SUCCESS;
 }

@@ -40,14 +40,15 @@ struct Loader {
     int32_t kidx[NLD];      // K index of each load for K-tile 0
     uint32_t kstep;         // bytes to advance per K-tile
     int32_t K;
+    int tid;                // thread index in the workgroup
     i32x4_t v[NLD];
 
     // ld: leading dimension (elements); rows: M or N; row0: first row of tile
     __device__ inline void init(const void* base, uint32_t extent_bytes, int32_t ld, int32_t rows,
-                                int32_t K_, int32_t row0) {
+                                int32_t K_, int32_t row0, int t) {
         rsrc = make_rsrc(base, extent_bytes);
         K = K_;
-        const int t = threadIdx.x;
+        tid = t;
         if constexpr (KC) {
             const int c = t & 7;
 #pragma unroll
@@ -85,7 +86,7 @@ struct Loader {
 
     // Write the loaded K-tile into one LDS stage (K-contiguous 128-byte rows).
     __device__ inline void store(char* stage) const {
-        const int t = threadIdx.x;
+        const int t = tid;
         if constexpr (KC) {
             const int c = t & 7;
 #pragma unroll
@@ -358,7 +359,7 @@ __device__ inline uint32_t extent_bytes(int64_t rows, int64_t cols_ld, int64_t t
 // One output tile.  smem: 2 stages x (MT_M + MT_N) x 128 bytes.
 // ---------------------------------------------------------------------------
 template <int MODE, bool AK, bool BK, int MT_M, int MT_N, int OUT>
-__device__ inline void gemm_tile(const TileArgs& a, int tm, int tn, int b, char* smem) {
+__device__ inline void gemm_tile(const TileArgs& a, int tm, int tn, int b, char* smem, int tid) {
     constexpr int ES = mode_elem_bytes(MODE);
     constexpr int DU = mode_depth_u(MODE);
     constexpr int WT_M = MT_M / 2, WT_N = MT_N / 2;
@@ -367,8 +368,8 @@ __device__ inline void gemm_tile(const TileArgs& a, int tm, int tn, int b, char*
     constexpr int STAGE = (MT_M + MT_N) * kRowBytes;
     static_assert(TM >= 1 && TN >= 1, "wave tile below one MFMA tile");
 
-    const int lane = threadIdx.x & 63;
-    const int wave = threadIdx.x >> 6;
+    const int lane = tid & 63;
+    const int wave = tid >> 6;
     const int wm = wave & 1, wn = wave >> 1;
     const int m0 = tm * MT_M, n0 = tn * MT_N;
 
@@ -378,8 +379,8 @@ __device__ inline void gemm_tile(const TileArgs& a, int tm, int tn, int b, char*
     Loader<MODE, AK, MT_M, true> la;
     Loader<MODE, BK, MT_N, false> lb;
     // A: K-contiguous = rows M of length lda; M-contiguous = rows K of length lda
-    la.init(Ab, AK ? extent_bytes(a.M, a.lda, a.K, ES) : extent_bytes(a.K, a.lda, a.M, ES), a.lda, a.M, a.K, m0);
-    lb.init(Bb, BK ? extent_bytes(a.N, a.ldb, a.K, ES) : extent_bytes(a.K, a.ldb, a.N, ES), a.ldb, a.N, a.K, n0);
+    la.init(Ab, AK ? extent_bytes(a.M, a.lda, a.K, ES) : extent_bytes(a.K, a.lda, a.M, ES), a.lda, a.M, a.K, m0, tid);
+    lb.init(Bb, BK ? extent_bytes(a.N, a.ldb, a.K, ES) : extent_bytes(a.K, a.ldb, a.N, ES), a.ldb, a.N, a.K, n0, tid);
 
     f32x16_t acc[TM][TN];
 #pragma unroll
@@ -498,7 +499,7 @@ __global__ void __launch_bounds__(kThreads, 1) gemm_kernel(TileArgs a) {
     __shared__ __attribute__((aligned(16))) char smem[2 * (MT_M + MT_N) * kRowBytes];
     int tb, tm, tn;
     map_tile(blockIdx.x, gridDim.x, a, tb, tm, tn);
-    gemm_tile<MODE, AK, BK, MT_M, MT_N, OUT>(a, tm, tn, tb, smem);
+    gemm_tile<MODE, AK, BK, MT_M, MT_N, OUT>(a, tm, tn, tb, smem, threadIdx.x);
 }
 
 }  // namespace ffgemm

@@ -82,10 +82,10 @@ struct DirectLoader {
     int wave;          // wave-uniform (SGPR)
 
     __device__ inline void init(const void* base, uint32_t extent, int32_t ld, int32_t rows, int32_t K,
-                                int32_t row0) {
+                                int32_t row0, int tid) {
         rsrc = make_rsrc(base, extent);
-        wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
-        const int lane = threadIdx.x & 63;
+        wave = __builtin_amdgcn_readfirstlane(tid >> 6);
+        const int lane = tid & 63;
 #pragma unroll
         for (int j = 0; j < NI; ++j) {
             const int lr = (wave * NI + j) * (64 / LPR) + lane / LPR;
@@ -128,9 +128,8 @@ struct TransLoader {
     i32x4_t v[PF][KB];
 
     __device__ inline void init(const void* base, uint32_t extent, int32_t ld, int32_t rows, int32_t K,
-                                int32_t row0) {
+                                int32_t row0, int t) {
         rsrc = make_rsrc(base, extent);
-        const int t = threadIdx.x;
         kg = t % NKG;
         mc = t / NKG;
         const int r = row0 + mc * 8;
@@ -216,7 +215,7 @@ __device__ inline void mfma16(const i32x4_t& a, const i32x4_t& b, f32x16_t& acc)
 }
 
 template <int MODE, bool AK, bool BK, int MT_M, int MT_N, int OUT>
-__device__ inline void gemm_tile(const TileArgs& a, int tm, int tn, int b, char* smem) {
+__device__ inline void gemm_tile(const TileArgs& a, int tm, int tn, int b, char* smem, int tid) {
     static_assert(MODE == kBF16 || MODE == kF16, "16-bit engine");
     using C = Cfg<MT_M, MT_N, AK, BK>;
     constexpr int RB = C::RB, DU = C::DU, RS = C::RS, NS = C::NS, PF = C::PF;
@@ -230,8 +229,8 @@ __device__ inline void gemm_tile(const TileArgs& a, int tm, int tn, int b, char*
     constexpr int NW = LA::NSTORE + LB::NSTORE;      // transposed LDS stores per K-tile
     static_assert(RS % 2 == 0, "read steps pair up in the fragment double buffer");
 
-    const int lane = threadIdx.x & 63;
-    const int wave = threadIdx.x >> 6;
+    const int lane = tid & 63;
+    const int wave = tid >> 6;
     const int wm = wave & 1, wn = wave >> 1;
     const int m0 = tm * MT_M, n0 = tn * MT_N;
     const int g = lane >> 5;
@@ -240,8 +239,8 @@ __device__ inline void gemm_tile(const TileArgs& a, int tm, int tn, int b, char*
     const char* Bb = (const char*)a.B + (int64_t)b * a.bsB * 2;
     LA la;
     LB lb;
-    la.init(Ab, AK ? extent_bytes(a.M, a.lda, a.K, 2) : extent_bytes(a.K, a.lda, a.M, 2), a.lda, a.M, a.K, m0);
-    lb.init(Bb, BK ? extent_bytes(a.N, a.ldb, a.K, 2) : extent_bytes(a.K, a.ldb, a.N, 2), a.ldb, a.N, a.K, n0);
+    la.init(Ab, AK ? extent_bytes(a.M, a.lda, a.K, 2) : extent_bytes(a.K, a.lda, a.M, 2), a.lda, a.M, a.K, m0, tid);
+    lb.init(Bb, BK ? extent_bytes(a.N, a.ldb, a.K, 2) : extent_bytes(a.K, a.ldb, a.N, 2), a.ldb, a.N, a.K, n0, tid);
 
     f32x16_t acc[TM][TN];
 #pragma unroll
@@ -352,7 +351,7 @@ __global__ void __launch_bounds__(kThreads, (Cfg<MT_M, MT_N, AK, BK>::OCC)) gemm
     __shared__ __attribute__((aligned(16))) char smem[smem_bytes<MT_M, MT_N, AK, BK>()];
     int tb, tm, tn;
     map_tile(blockIdx.x, gridDim.x, a, tb, tm, tn);
-    v2::gemm_tile<MODE, AK, BK, MT_M, MT_N, OUT>(a, tm, tn, tb, smem);
+    v2::gemm_tile<MODE, AK, BK, MT_M, MT_N, OUT>(a, tm, tn, tb, smem, threadIdx.x);
 }
 
 }  // namespace v2

@@ -47,12 +47,14 @@ struct RegLoader {
     __amdgpu_buffer_rsrc_t rsrc;
     uint32_t voff[NLD];
     int32_t klim;
+    int tid;  // thread index in the workgroup
     i32x4_t v[PF][NLD];
 
     __device__ inline void init(const void* base, uint32_t extent, int32_t ld, int32_t rows, int32_t K,
-                                int32_t row0) {
+                                int32_t row0, int t) {
         rsrc = make_rsrc(base, extent);
-        const int t = threadIdx.x, c = t & 7;
+        tid = t;
+        const int c = t & 7;
         klim = K - c * 8;
 #pragma unroll
         for (int i = 0; i < NLD; ++i) {
@@ -69,7 +71,7 @@ struct RegLoader {
     }
     template <int S>
     __device__ inline void store(char* region) const {
-        const int t = threadIdx.x, c = t & 7;
+        const int t = tid, c = t & 7;
 #pragma unroll
         for (int i = 0; i < NLD; ++i) {
             const int r = (t >> 3) + 32 * i;
@@ -82,7 +84,7 @@ template <int NM, int NR, int SYNC, int MASK>
 __device__ inline void sched_block(int nx) { v2::sched_block<NM, NR, SYNC, MASK>(nx); }
 
 template <int MODE, bool AK, bool BK, int OUT>
-__device__ inline void gemm_tile(const TileArgs& a, int tm, int tn, int b, char* smem) {
+__device__ inline void gemm_tile(const TileArgs& a, int tm, int tn, int b, char* smem, int tid) {
     static_assert(MODE == kBF16 || MODE == kF16, "16-bit engine");
     constexpr int WT = MT / 2, T = WT / 32;  // 128 x 128 per wave, 4 x 4 MFMA tiles
     // A: register ring of 2 slots (two K-tiles ahead) -> 2 LDS stages
@@ -92,8 +94,8 @@ __device__ inline void gemm_tile(const TileArgs& a, int tm, int tn, int b, char*
     constexpr int NV = LA::NVMEM + LB::NVMEM;
     constexpr int NW = LA::NSTORE + LB::NSTORE;
 
-    const int lane = threadIdx.x & 63;
-    const int wave = threadIdx.x >> 6;
+    const int lane = tid & 63;
+    const int wave = tid >> 6;
     const int wm = wave & 1, wn = wave >> 1;
     const int m0 = tm * MT, n0 = tn * MT;
     const int g = lane >> 5;
@@ -102,8 +104,8 @@ __device__ inline void gemm_tile(const TileArgs& a, int tm, int tn, int b, char*
     const char* Bb = (const char*)a.B + (int64_t)b * a.bsB * 2;
     LA la;
     LB lb;
-    la.init(Ab, AK ? extent_bytes(a.M, a.lda, a.K, 2) : extent_bytes(a.K, a.lda, a.M, 2), a.lda, a.M, a.K, m0);
-    lb.init(Bb, BK ? extent_bytes(a.N, a.ldb, a.K, 2) : extent_bytes(a.K, a.ldb, a.N, 2), a.ldb, a.N, a.K, n0);
+    la.init(Ab, AK ? extent_bytes(a.M, a.lda, a.K, 2) : extent_bytes(a.K, a.lda, a.M, 2), a.lda, a.M, a.K, m0, tid);
+    lb.init(Bb, BK ? extent_bytes(a.N, a.ldb, a.K, 2) : extent_bytes(a.K, a.ldb, a.N, 2), a.ldb, a.N, a.K, n0, tid);
 
     f32x16_t acc[T][T];
 #pragma unroll
@@ -212,7 +214,7 @@ __global__ void __launch_bounds__(kThreads, 1) gemm_kernel(TileArgs a) {
     __shared__ __attribute__((aligned(16))) char smem[LDS_BYTES];
     int tb, tm, tn;
     map_tile(blockIdx.x, gridDim.x, a, tb, tm, tn);
-    v3::gemm_tile<MODE, AK, BK, OUT>(a, tm, tn, tb, smem);
+    v3::gemm_tile<MODE, AK, BK, OUT>(a, tm, tn, tb, smem, threadIdx.x);
 }
 
 }  // namespace v3

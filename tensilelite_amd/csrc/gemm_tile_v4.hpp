@@ -65,9 +65,9 @@ struct HalfLoader {
     int32_t lds_off;  // this wave's first LDS row, in bytes
 
     __device__ inline void init(const void* ptr, uint32_t extent, int32_t ld, int32_t rows, int32_t K_,
-                                int32_t row0, int h, int s) {
+                                int32_t row0, int h, int s, int tid) {
         rsrc = make_rsrc(ptr, extent);
-        const int lane = threadIdx.x & 63;
+        const int lane = tid & 63;
         const int j = lane >> 3;
         const int lr0 = 128 * h + 32 * s + j;  // LDS row of q = 0
         const int r = row0 + (PERM ? v2::perm_a_inv(lr0) : lr0);
@@ -106,9 +106,10 @@ struct MNHalfLoader {
     uint32_t kstride;  // 64 k
     int32_t lds_off;   // this wave's first k, in bytes of the region
 
-    __device__ inline void init(const void* ptr, uint32_t extent, int32_t ld, int32_t K, int32_t row0, int s) {
+    __device__ inline void init(const void* ptr, uint32_t extent, int32_t ld, int32_t K, int32_t row0, int s,
+                                int tid) {
         rsrc = make_rsrc(ptr, extent);
-        const int lane = threadIdx.x & 63;
+        const int lane = tid & 63;
         const int kr = 16 * s + (lane >> 4);
         const int c = (lane & 15) ^ (4 * (lane >> 4));
         base = (uint32_t)(((int64_t)kr * ld + row0 + 8 * c) * 2);
@@ -164,13 +165,13 @@ __device__ inline void mfma_f8(const i32x4_t& a0, const i32x4_t& a1, const i32x4
 }
 
 template <int MODE, bool AK, bool BK, int OUT>
-__device__ inline void gemm_tile(const TileArgs& a, int tm, int tn, int b, char* smem) {
+__device__ inline void gemm_tile(const TileArgs& a, int tm, int tn, int b, char* smem, int tid) {
     static_assert(MODE == kBF16 || MODE == kF16 || MODE == kF8, "16-bit or fp8 engine");
     static_assert((AK && BK) || MODE != kF8, "fp8 operands are K-contiguous");
     constexpr int ES = (MODE == kF8) ? 1 : 2;
     constexpr int DU = RB / ES;
-    const int lane = threadIdx.x & 63;
-    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const int lane = tid & 63;
+    const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
     const int grp = wave >> 2, s = wave & 3;
     const int h = lane >> 5;
     const int m0 = tm * MT, n0 = tn * MT;
@@ -183,10 +184,10 @@ __device__ inline void gemm_tile(const TileArgs& a, int tm, int tn, int b, char*
     using LB = std::conditional_t<BK, HalfLoader<false, ES>, MNHalfLoader>;
     LA la;
     LB lb;
-    if constexpr (AK) la.init(Ab, extent_bytes(a.M, a.lda, a.K, ES), a.lda, a.M, a.K, m0, grp, s);
-    else la.init(Ab, extent_bytes(a.K, a.lda, a.M, 2), a.lda, a.K, m0 + 128 * grp, s);
-    if constexpr (BK) lb.init(Bb, extent_bytes(a.N, a.ldb, a.K, ES), a.ldb, a.N, a.K, n0, grp, s);
-    else lb.init(Bb, extent_bytes(a.K, a.ldb, a.N, 2), a.ldb, a.K, n0 + 128 * grp, s);
+    if constexpr (AK) la.init(Ab, extent_bytes(a.M, a.lda, a.K, ES), a.lda, a.M, a.K, m0, grp, s, tid);
+    else la.init(Ab, extent_bytes(a.K, a.lda, a.M, 2), a.lda, a.K, m0 + 128 * grp, s, tid);
+    if constexpr (BK) lb.init(Bb, extent_bytes(a.N, a.ldb, a.K, ES), a.ldb, a.N, a.K, n0, grp, s, tid);
+    else lb.init(Bb, extent_bytes(a.K, a.ldb, a.N, 2), a.ldb, a.K, n0 + 128 * grp, s, tid);
     // where this group's loads go inside a stage (K-contiguous loaders add it themselves)
     const int lhalf_a = AK ? 0 : 16384 * grp, lhalf_b = BK ? 0 : 16384 * grp;
 
@@ -324,7 +325,7 @@ __global__ void __launch_bounds__(kThreads8, 1) gemm_kernel(TileArgs a) {
     __shared__ __attribute__((aligned(16))) char smem[LDS_BYTES];
     int tb, tm, tn;
     map_tile(blockIdx.x, gridDim.x, a, tb, tm, tn);
-    v4::gemm_tile<MODE, AK, BK, OUT>(a, tm, tn, tb, smem);
+    v4::gemm_tile<MODE, AK, BK, OUT>(a, tm, tn, tb, smem, threadIdx.x);
 }
 
 }  // namespace v4

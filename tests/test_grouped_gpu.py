@@ -45,12 +45,49 @@ def test_grouped_uneven_shapes(cuda, tile):
     plan.close()
 
 
-def test_grouped_config_graph_replays_match_eager(cuda):
+# Uneven problems with ragged K (not a multiple of 64) whose tiles outnumber the
+# workgroups of the persistent grid at every tile below (256 x 256: 265 tiles
+# on 256 workgroups; 128 x 256: 523 on 256 or 512, by layout), so workgroups run several tiles of
+# different problems one after the other.
+_LAYOUT_SHAPES = [(4096, 2048, 64), (3000, 1000, 200), (2048, 2048, 96), (72, 256, 512), (1000, 304, 64),
+                  (8, 40, 24), (520, 1032, 328)]
+
+
+@pytest.mark.parametrize("layout", ["NN", "NT", "TN", "TT"])
+@pytest.mark.parametrize("tile,dtype", [((256, 256), torch.bfloat16), ((128, 256), torch.bfloat16),
+                                        ((256, 256), torch.float16)])
+def test_grouped_layouts_persistent(cuda, layout, tile, dtype):
+    print(f"case grouped {layout} tile={tile} {dtype}", flush=True)
+    ta, tb = layout[0] == "T", layout[1] == "T"
+    g = torch.Generator().manual_seed(11)
+    probs, As, Bs, Ds, lds = [], [], [], [], []
+    for (M, N, K) in _LAYOUT_SHAPES:
+        lda, ldb = (K if ta else M), (N if tb else K)
+        A = (torch.randn(M * K, generator=g) / K ** 0.5).to(dtype)
+        B = (torch.randn(K * N, generator=g) / K ** 0.5).to(dtype)
+        As.append(A.to(cuda)); Bs.append(B.to(cuda))
+        Ds.append(torch.full((M * N,), 7.0, dtype=dtype, device=cuda))
+        probs.append(tl.make_problem(layout, dtype, M, N, K, lda, ldb, M, M))
+        lds.append((lda, ldb))
+    plan = grouped.GroupedPlan(probs, As, Bs, None, Ds, tile=tile)
+    info = plan.info()
+    assert info["tile"] == tile and info["total_tiles"] > info["grid"], info
+    plan.run()
+    torch.cuda.synchronize()
+    for (M, N, K), (lda, ldb), A, B, D in zip(_LAYOUT_SHAPES, lds, As, Bs, Ds):
+        ref = oracle.gemm_f64(layout, M, N, K, 1.0, A.double().cpu().numpy(), lda, B.double().cpu().numpy(), ldb)
+        bad, worst = elem_ok(D.float().cpu().numpy(), ref)
+        assert bad == 0, (layout, M, N, K, worst)
+    plan.close()
+
+
+@pytest.mark.parametrize("tile", [(128, 128), (256, 256)])
+def test_grouped_config_graph_replays_match_eager(cuda, tile):
     """The batching configuration (64 problems, seed 3) captured in a hipGraph:
     every replay, with eager work in between, equals the eager launch."""
-    print("case grouped config + graph", flush=True)
+    print(f"case grouped config + graph tile={tile}", flush=True)
     probs, As, Bs, Ds = grouped.make_config_problems(cuda, count=64)
-    plan = grouped.GroupedPlan(probs, As, Bs, None, Ds, tile=(128, 128))
+    plan = grouped.GroupedPlan(probs, As, Bs, None, Ds, tile=tile)
     plan.run()
     torch.cuda.synchronize()
     eager = [d.clone() for d in Ds]
