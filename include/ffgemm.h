@@ -155,10 +155,13 @@ int ffgemm_fp8_quantize(const float *x, int64_t rows, int64_t k, int64_t ldx, in
 int ffgemm_convert(const float *src, void *dst, int64_t n, int dtype_dst, void *stream);
 
 /* Grouped GEMM: `count` independent problems (same dtype, layout, compute)
- * run by one persistent launch.  A plan uploads the problem descriptors once
- * (this call allocates device memory and synchronises), after which
- * ffgemm_grouped_plan_run only launches a kernel and can be captured into a
- * hipGraph.  Pointer arrays are host arrays of device pointers. */
+ * run by one persistent launch.  A plan uploads the problem descriptors and
+ * the tile schedule (every output tile dealt to a workgroup of the grid,
+ * longest K first) once; this call allocates device memory and synchronises.
+ * After it, ffgemm_grouped_plan_run only launches a kernel and can be
+ * captured into a hipGraph.  `tile` picks the macro tile (NULL: the largest
+ * with at least two tiles per CU); each tile runs the default engine for
+ * it.  Pointer arrays are host arrays of device pointers. */
 typedef struct ffgemm_grouped_plan_s *ffgemm_grouped_plan;
 int ffgemm_grouped_plan_create(ffgemm_grouped_plan *plan, int count, const ffgemm_problem *probs,
                                const void *const *A, const void *const *B,
