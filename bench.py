@@ -254,6 +254,39 @@ def run_extras(a, torch, np, tl, C, dev):
                                               "fwd_ms": round(ms_f, 4), "fwd_bwd_ms": round(ms_fb, 4)}
         del a_, b_, dc
 
+    # small shapes, where launch and latency dominate: the reference's analysed
+    # kernel shape (fp16 TN 480x512x512) and its CPU case on the GPU (fp32 NN
+    # 480x512x512).  Timed eagerly (each launch goes through Python and ctypes)
+    # and as a hipGraph of 100 launches replayed back to back.
+    if want("small_480"):
+        res = {}
+        m, n, k = 480, 512, 512
+        for name, layout, tdt in (("f16_TN", "TN", torch.float16), ("f32_NN", "NN", torch.float32)):
+            A = (torch.rand(m * k, generator=g) * 2 - 1).to(tdt).to(dev)
+            B = (torch.rand(k * n, generator=g) * 2 - 1).to(tdt).to(dev)
+            D = torch.empty(m * n, dtype=tdt, device=dev)
+            p = tl.make_problem(layout, tdt, m, n, k, k if layout[0] == "T" else m, k, m, m)
+            fn = lambda: tl.run(p, A, B, None, D)
+            ms_eager = time_launches(fn, 200, 20, torch)
+            side = torch.cuda.Stream()
+            side.wait_stream(torch.cuda.current_stream())
+            with torch.cuda.stream(side):
+                fn()
+            torch.cuda.current_stream().wait_stream(side)
+            torch.cuda.synchronize()
+            graph = torch.cuda.CUDAGraph()
+            with torch.cuda.graph(graph):
+                for _ in range(100):
+                    fn()
+            ms_graph = time_launches(graph.replay, 10, 2, torch) / 100
+            fl = 2.0 * m * n * k
+            roof_us = max(fl / (2500e12 if tdt != torch.float32 else 157.3e12),
+                          (m * k + k * n + m * n) * A.element_size() / 8e12) * 1e6
+            res[name] = {"us_eager": round(ms_eager * 1e3, 2), "us_graph": round(ms_graph * 1e3, 2),
+                         "tflops_graph": round(fl / (ms_graph * 1e-3) / 1e12, 2), "roofline_us": round(roof_us, 3)}
+            del A, B, D, graph
+        ex["small_480x512x512"] = res
+
     # grouped: 64 problems, one persistent launch, hipGraph replay
     if want("grouped_64"):
         from tensilelite_amd import grouped

@@ -14,9 +14,10 @@ namespace ffgemm {
 // the default, what measured fastest on MI355X at 4096^3: the eight-wave
 // ping-pong engine (v4::) for 256 x 256 TN and NT (M/N-contiguous operands
 // through transposing LDS reads) and fp8 TN; the split-ring engine (v3::) for
-// 256 x 256 NN; the pipelined 4-wave engine (v2::) for 256 x 256 TT and the
-// other 16-bit tiles of 128 and up; the first engine (gemm_tile.hpp) for the
-// rest and for fp32 / bf16x3.  waves = 8 or 4 asks for the eight-wave or a
+// 256 x 256 NN; the pipelined 4-wave engine (v2::) for 256 x 256 TT, the
+// other 16-bit tiles of 128 and up, and 16-bit 64 x 64 TN (its three-stage
+// direct-load ring shortens the latency chain of small problems); the first
+// engine (gemm_tile.hpp) for the rest and for fp32 / bf16x3.  waves = 8 or 4 asks for the eight-wave or a
 // four-wave engine (the wave-tiling knob of the tile record, which the
 // autotuner sweeps); kEngNone when the combination has no kernel.
 enum Engine : int { kEngNone = 0, kEngV1 = 1, kEngV2 = 2, kEngV3 = 3, kEngV4 = 4 };
@@ -24,7 +25,8 @@ __host__ __device__ constexpr Engine engine_for(int mode, bool ak, bool bk, int 
     const bool k16 = (mode == kBF16 || mode == kF16);
     const bool big = (tm == 256 && tn == 256);
     const bool v4ok = big && (k16 || (mode == kF8 && ak && bk));
-    const Engine four = (big && k16 && !ak && bk) ? kEngV3 : ((k16 && tm >= 128 && tn >= 128) ? kEngV2 : kEngV1);
+    const bool v2ok = k16 && ((tm >= 128 && tn >= 128) || (tm == 64 && tn == 64 && ak && bk));
+    const Engine four = (big && k16 && !ak && bk) ? kEngV3 : (v2ok ? kEngV2 : kEngV1);
     if (waves == 8) return v4ok ? kEngV4 : kEngNone;
     if (waves == 4) return four;
     if (waves != 0) return kEngNone;
@@ -48,7 +50,7 @@ inline EngineGeometry engine_geometry(int mode, bool ak, bool bk, int tm, int tn
     if (tm == TM_ && tn == TN_ && ak == AK_ && bk == BK_) { ns = v2::Cfg<TM_, TN_, AK_, BK_>::NS; du = v2::Cfg<TM_, TN_, AK_, BK_>::DU; }
 #define FFGEMM_V2L(TM_, TN_) FFGEMM_V2G(TM_, TN_, false, false) FFGEMM_V2G(TM_, TN_, false, true) \
     FFGEMM_V2G(TM_, TN_, true, false) FFGEMM_V2G(TM_, TN_, true, true)
-            FFGEMM_V2L(256, 256) FFGEMM_V2L(256, 128) FFGEMM_V2L(128, 256) FFGEMM_V2L(128, 128)
+            FFGEMM_V2L(256, 256) FFGEMM_V2L(256, 128) FFGEMM_V2L(128, 256) FFGEMM_V2L(128, 128) FFGEMM_V2L(64, 64)
 #undef FFGEMM_V2L
 #undef FFGEMM_V2G
             return {2, 2, du, ns};
