@@ -104,16 +104,27 @@ __global__ void __launch_bounds__(256) x3_split_kernel(const float* __restrict__
     uint16_t* ob = out + (int64_t)blockIdx.z * rows * 3 * kp;
     const int t = threadIdx.x;
     const int kc = t & 7;  // this thread's 8-element K chunk
+    // 16-byte loads where the operand's base and leading dimension keep them aligned
+    const bool vec = (((uintptr_t)xb | (uintptr_t)(ldx * 4)) & 15) == 0;
     if (!trans_x) {
-        // M-contiguous: 64 kk columns of 64 rows, read along the rows
+        // M-contiguous: 64 kk columns of 64 rows, each thread 4 consecutive rows of one kk
+        const bool rows_in = vec && r0 + 64 <= rows;
         for (int i = t; i < 64 * 16; i += 256) {
             const int kk = i >> 4, rq = (i & 15) * 4;
             const int64_t kg = k0 + kk;
+            float v[4] = {0.0f, 0.0f, 0.0f, 0.0f};
+            if (kg < k) {
+                const float* src = xb + kg * ldx + r0 + rq;
+                if (rows_in) {
+                    const float4 q = *(const float4*)src;
+                    v[0] = q.x; v[1] = q.y; v[2] = q.z; v[3] = q.w;
+                } else {
 #pragma unroll
-            for (int e = 0; e < 4; ++e) {
-                const int64_t rg = r0 + rq + e;
-                tile[rq + e][kk] = (rg < rows && kg < k) ? xb[kg * ldx + rg] : 0.0f;
+                    for (int e = 0; e < 4; ++e) v[e] = (r0 + rq + e < rows) ? src[e] : 0.0f;
+                }
             }
+#pragma unroll
+            for (int e = 0; e < 4; ++e) tile[rq + e][kk] = v[e];
         }
         __syncthreads();
     }
@@ -125,8 +136,14 @@ __global__ void __launch_bounds__(256) x3_split_kernel(const float* __restrict__
         float v[8];
         if (trans_x) {
             const float* src = xb + r * ldx + kk;
+            if (vec && kk + 8 <= k) {
+                const float4 q0 = *(const float4*)src, q1 = *(const float4*)(src + 4);
+                v[0] = q0.x; v[1] = q0.y; v[2] = q0.z; v[3] = q0.w;
+                v[4] = q1.x; v[5] = q1.y; v[6] = q1.z; v[7] = q1.w;
+            } else {
 #pragma unroll
-            for (int e = 0; e < 8; ++e) v[e] = (kk + e < k) ? src[e] : 0.0f;
+                for (int e = 0; e < 8; ++e) v[e] = (kk + e < k) ? src[e] : 0.0f;
+            }
         } else {
 #pragma unroll
             for (int e = 0; e < 8; ++e) v[e] = tile[rl][kc * 8 + e];
