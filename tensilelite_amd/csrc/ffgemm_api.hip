@@ -189,10 +189,13 @@ int prepare(const ffgemm_problem* p, const void* A, const void* B, const void* C
         if ((p->bsA * es) % 16 || (p->bsB * es) % 16 || (p->bsD * os) % 16 || (use_c && (p->bsC * os) % 16))
             return fail(FFGEMM_ERR, "batch strides must keep 16-byte alignment");
     }
-    // every operand must be addressable with 31-bit byte offsets
-    const int64_t exA = ((ta ? M : K) + 256) * p->lda * es;
-    const int64_t exB = ((tb ? K : N) + 256) * p->ldb * es;
-    if (exA >= lim || exB >= lim) return fail(FFGEMM_ERR, "operand larger than 2 GiB per batch");
+    // Buffer offsets are 31-bit.  A K-contiguous operand is addressed from each
+    // tile's first row (at most 256 rows, and a margin row), so any number of
+    // rows fits; an M/N-contiguous operand is addressed whole, all K rows of it.
+    const int64_t exA = (ta ? 257 : K + 256) * p->lda * es;
+    const int64_t exB = (tb ? K + 256 : 257) * p->ldb * es;
+    if (exA >= lim || exB >= lim)
+        return fail(FFGEMM_ERR, "operand exceeds 2 GiB per batch (K-contiguous operands: 257 rows of ld)");
 
     Tile t = pick_tile(M, N, batch);
     int group_m = 0;
@@ -350,9 +353,9 @@ bool x3_planes_apply(const ffgemm_problem* p) {
     if (p->M <= 0 || p->N <= 0 || p->K <= 0) return false;
     const int64_t batch = p->batch > 0 ? p->batch : 1;
     const int64_t row = 3 * x3_kp(p->K) * 2;  // bytes per plane row
-    // plane extents pass the bf16 GEMM's own 31-bit offset check (which allows
-    // 256 rows of tile overhang); split grid limits
-    return (p->M + 256) * row < (1ll << 31) && (p->N + 256) * row < (1ll << 31) &&
+    // the planes are K-contiguous operands of the bf16 GEMM, addressed from each
+    // tile's first row (its 31-bit offset check); split grid limits
+    return 257 * row < (1ll << 31) &&
            cdiv(std::max(p->M, p->N), 64) <= 65535 &&
            batch <= 65535;
 }

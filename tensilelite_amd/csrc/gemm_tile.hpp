@@ -379,8 +379,12 @@ __device__ inline void gemm_tile(const TileArgs& a, int tm, int tn, int b, char*
     Loader<MODE, AK, MT_M, true> la;
     Loader<MODE, BK, MT_N, false> lb;
     // A: K-contiguous = rows M of length lda; M-contiguous = rows K of length lda
-    la.init(Ab, AK ? extent_bytes(a.M, a.lda, a.K, ES) : extent_bytes(a.K, a.lda, a.M, ES), a.lda, a.M, a.K, m0, tid);
-    lb.init(Bb, BK ? extent_bytes(a.N, a.ldb, a.K, ES) : extent_bytes(a.K, a.ldb, a.N, ES), a.ldb, a.N, a.K, n0, tid);
+    // A K-contiguous operand's buffer starts at the tile's first row, so its
+    // offsets stay within the tile's rows however large the matrix is.
+    if constexpr (AK) la.init(Ab + (int64_t)m0 * a.lda * ES, extent_bytes(a.M - m0, a.lda, a.K, ES), a.lda, a.M - m0, a.K, 0, tid);
+    else la.init(Ab, extent_bytes(a.K, a.lda, a.M, ES), a.lda, a.M, a.K, m0, tid);
+    if constexpr (BK) lb.init(Bb + (int64_t)n0 * a.ldb * ES, extent_bytes(a.N - n0, a.ldb, a.K, ES), a.ldb, a.N - n0, a.K, 0, tid);
+    else lb.init(Bb, extent_bytes(a.K, a.ldb, a.N, ES), a.ldb, a.N, a.K, n0, tid);
 
     f32x16_t acc[TM][TN];
 #pragma unroll

@@ -104,8 +104,11 @@ __device__ inline void gemm_tile(const TileArgs& a, int tm, int tn, int b, char*
     const char* Bb = (const char*)a.B + (int64_t)b * a.bsB * 2;
     LA la;
     LB lb;
-    la.init(Ab, AK ? extent_bytes(a.M, a.lda, a.K, 2) : extent_bytes(a.K, a.lda, a.M, 2), a.lda, a.M, a.K, m0, tid);
-    lb.init(Bb, BK ? extent_bytes(a.N, a.ldb, a.K, 2) : extent_bytes(a.K, a.ldb, a.N, 2), a.ldb, a.N, a.K, n0, tid);
+    // K-contiguous operands: the buffer starts at the tile's first row
+    if constexpr (AK) la.init(Ab + (int64_t)m0 * a.lda * 2, extent_bytes(a.M - m0, a.lda, a.K, 2), a.lda, a.M - m0, a.K, 0, tid);
+    else la.init(Ab, extent_bytes(a.K, a.lda, a.M, 2), a.lda, a.M, a.K, m0, tid);
+    if constexpr (BK) lb.init(Bb + (int64_t)n0 * a.ldb * 2, extent_bytes(a.N - n0, a.ldb, a.K, 2), a.ldb, a.N - n0, a.K, 0, tid);
+    else lb.init(Bb, extent_bytes(a.K, a.ldb, a.N, 2), a.ldb, a.N, a.K, n0, tid);
 
     f32x16_t acc[T][T];
 #pragma unroll

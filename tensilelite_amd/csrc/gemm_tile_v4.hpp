@@ -184,9 +184,13 @@ __device__ inline void gemm_tile(const TileArgs& a, int tm, int tn, int b, char*
     using LB = std::conditional_t<BK, HalfLoader<false, ES>, MNHalfLoader>;
     LA la;
     LB lb;
-    if constexpr (AK) la.init(Ab, extent_bytes(a.M, a.lda, a.K, ES), a.lda, a.M, a.K, m0, grp, s, tid);
+    // K-contiguous operands: the buffer starts at the tile's first row, so the
+    // 32-bit offsets stay within the tile's 256 rows however large the matrix
+    if constexpr (AK)
+        la.init(Ab + (int64_t)m0 * a.lda * ES, extent_bytes(a.M - m0, a.lda, a.K, ES), a.lda, a.M - m0, a.K, 0, grp, s, tid);
     else la.init(Ab, extent_bytes(a.K, a.lda, a.M, 2), a.lda, a.K, m0 + 128 * grp, s, tid);
-    if constexpr (BK) lb.init(Bb, extent_bytes(a.N, a.ldb, a.K, ES), a.ldb, a.N, a.K, n0, grp, s, tid);
+    if constexpr (BK)
+        lb.init(Bb + (int64_t)n0 * a.ldb * ES, extent_bytes(a.N - n0, a.ldb, a.K, ES), a.ldb, a.N - n0, a.K, 0, grp, s, tid);
     else lb.init(Bb, extent_bytes(a.K, a.ldb, a.N, 2), a.ldb, a.K, n0 + 128 * grp, s, tid);
     // where this group's loads go inside a stage (K-contiguous loaders add it themselves)
     const int lhalf_a = AK ? 0 : 16384 * grp, lhalf_b = BK ? 0 : 16384 * grp;

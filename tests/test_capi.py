@@ -77,6 +77,23 @@ def test_query_tile_rejects_with_rtn_err(bad, msg):
     assert msg in str(e.value)
 
 
+def test_operand_size_limits():
+    """Buffer offsets are 31-bit: a K-contiguous operand is addressed from each
+    tile's first row, so it may exceed 2 GiB; an M/N-contiguous one may not."""
+    M, N, K = 140000, 256, 8192                   # A: 140000 rows of 8192 bf16 = 2.29 GB
+    t = tl.query_tile(_p("TN", C.BF16, M, N, K))  # A and B K-contiguous
+    assert t.mt_m > 0
+    with pytest.raises(tl.FFGemmError) as e:      # NN: A is M-contiguous, K rows of lda = M
+        tl.query_tile(_p("NN", C.BF16, M, N, K))
+    assert e.value.status == C.ERR and "2 GiB" in str(e.value)
+    Kw = 4194304                                  # 257 rows of 2^22 bf16 exceed 2 GiB
+    with pytest.raises(tl.FFGemmError):
+        tl.query_tile(_p("TN", C.BF16, 512, 512, Kw))
+    # bf16x3 planes are K-contiguous too: a workspace is offered past 2 GiB of planes
+    x3 = tl.make_problem("TN", C.F32, M, N, K, K, K, M, M, compute=C.COMPUTE_BF16X3)
+    assert tl.workspace_size(x3) == ((M * 3 * K * 2 + 255) // 256 + (N * 3 * K * 2 + 255) // 256) * 256
+
+
 def test_bad_tile_request_is_rejected():
     with pytest.raises(tl.FFGemmError) as e:
         tl.query_tile(_p(), tile=(96, 96))

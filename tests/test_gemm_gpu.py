@@ -159,6 +159,53 @@ def test_beta_zero_never_reads_c(cuda):
     assert torch.isfinite(D).all()
 
 
+@pytest.mark.parametrize("layout", LAYOUTS)
+@pytest.mark.parametrize("tile", [(256, 256), (128, 128), (64, 64)])
+def test_16bit_beta_zero_never_reads_c(cuda, layout, tile):
+    """beta = 0 on every 16-bit engine: a NaN-filled C must not reach D."""
+    M, N, K = 264, 200, 96
+    _say(f"case bf16 beta0 NaN C {layout} tile={tile}")
+    rng = np.random.default_rng(12)
+    (ra, ca), (rb, cb) = op_dims(layout, M, N, K)
+    Ab = quantize_bits(colmajor(rng, ra, ca, kind="normal", scale=1 / np.sqrt(K)), "bf16")
+    Bb = quantize_bits(colmajor(rng, rb, cb, kind="normal", scale=1 / np.sqrt(K)), "bf16")
+    Cm = torch.full((M * N,), float("nan"), dtype=torch.bfloat16, device=cuda)
+    D = tl.gemm(layout, M, N, K, _dev16(Ab, "bf16", cuda), _dev16(Bb, "bf16", cuda), Cm, beta=0.0, tile=tile)
+    torch.cuda.synchronize()
+    got = D.float().cpu().numpy()
+    assert np.isfinite(got).all()
+    ref = _f64_ref(layout, M, N, K, bits_to_f32(Ab, "bf16"), ra, bits_to_f32(Bb, "bf16"), rb)
+    bad, worst = elem_ok(got, ref)
+    assert bad == 0, worst
+
+
+def test_tn_operand_past_2gib(cuda):
+    """bf16 TN with A of 140000 rows x K = 8192 (2.29 GB): rows whose bytes lie
+    past 2 GiB (row 131072 on) are computed from the rebased per-tile buffer;
+    sampled rows on both sides of the boundary are checked against the oracle."""
+    M, N, K = 140000, 256, 8192
+    _say(f"case TN operand past 2 GiB M={M} N={N} K={K}")
+    g = torch.Generator(device=cuda).manual_seed(13)
+    A = torch.empty(M * K, dtype=torch.bfloat16, device=cuda)
+    for r0 in range(0, M, 20000):  # fill in slices to bound the fp32 temporary
+        r1 = min(M, r0 + 20000)
+        A[r0 * K:r1 * K] = (torch.randn((r1 - r0) * K, generator=g, device=cuda) / K ** 0.5).to(torch.bfloat16)
+    B = (torch.randn(N * K, generator=g, device=cuda) / K ** 0.5).to(torch.bfloat16)
+    D = tl.gemm("TN", M, N, K, A, B)
+    torch.cuda.synchronize()
+    rows = [0, 1, 255, 256, 131071, 131072, 131073, 131327, 131328, 139743, 139999]
+    rows += torch.randint(0, M, (21,), generator=torch.Generator().manual_seed(14)).tolist()
+    rows = sorted(set(rows))
+    Ar = A.view(M, K)[rows].float().cpu().numpy().astype(np.float64).ravel()
+    Bh = B.float().cpu().numpy().astype(np.float64)
+    ref = oracle.gemm_f64("TN", len(rows), N, K, 1.0, Ar, K, Bh, K).reshape(N, len(rows))
+    got = D.view(N, M)[:, rows].float().cpu().numpy()
+    bad, worst = elem_ok(got, ref)
+    assert bad == 0, worst
+    del A, B, D
+    torch.cuda.empty_cache()
+
+
 def test_k_zero_and_alpha_zero(cuda):
     M, N = 96, 80
     _say("case K=0 / alpha=0")
