@@ -203,8 +203,18 @@ def run_extras(a, torch, np, tl, C, dev):
         p = tl.make_problem("TN", C.F8E4M3, S, S, S, S, S, S, S, scaleA=sa, scaleB=sb, dtype_out=C.F32)
         ms = time_launches(lambda: tl.run(p, qa, qb, None, D), max(5, steps // 4), 3, torch)
         tf = 2.0 * S ** 3 / (ms * 1e-3) / 1e12
-        ex[name] = {"tflops": round(tf, 1), "ms": round(ms, 4), "pct_peak": round(100 * tf / 5000.0, 1)}
-        del x, y, qa, qb, D
+        # error of the whole form (quantisation included) against the FP64 GPU
+        # product of the unquantised fp32 inputs, on 256 output columns: D[m, n]
+        # = sum_k x[m + k*S] * y[k + n*S], an NN product of x and y
+        cols = 256
+        Dref = torch.empty(S * cols, dtype=torch.float64, device=dev)
+        p64 = tl.make_problem("NN", C.F64, S, cols, S, S, S, S, S)
+        tl.run(p64, x.double(), y[: S * cols].double(), None, Dref)
+        torch.cuda.synchronize()
+        err = (torch.linalg.norm(D[: S * cols].double() - Dref) / torch.linalg.norm(Dref)).item()
+        ex[name] = {"tflops": round(tf, 1), "ms": round(ms, 4), "pct_peak": round(100 * tf / 5000.0, 1),
+                    "rel_fro_err_vs_fp64": err}
+        del x, y, qa, qb, D, Dref
 
     # precision: fp32 MFMA vs bf16x3 at 8192^3 (A, B ~ normal(0,1) seed 2)
     S = 8192

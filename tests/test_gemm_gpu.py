@@ -347,7 +347,11 @@ def test_fp8_quantize_bit_exact_and_gemm(cuda):
     # the products of e4m3 values are exact; the bound covers the f8f6f4 MFMA's
     # internal accumulation (measured 1.5e-5 here) and the fp32 dequant multiply
     assert err <= 5e-5
-    assert err_vs_fp32_inputs < 0.1
+    # the form's total error is the quantisation's own: the oracle's dequantised
+    # FP64 product has the same error against the fp32 inputs (e4m3: a few %)
+    err_q = rel_fro(ref, _f64_ref("NN", M, N, K, A, M, B, K))
+    assert 0.01 < err_q < 0.1
+    assert abs(err_vs_fp32_inputs - err_q) <= 1e-4
 
 
 @pytest.mark.parametrize("tile", [(256, 256), (128, 128), (64, 64)])
