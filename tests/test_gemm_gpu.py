@@ -224,6 +224,24 @@ def test_k_zero_and_alpha_zero(cuda):
     np.testing.assert_array_equal(D.cpu().numpy(), Cm)
 
 
+@pytest.mark.parametrize("layout", LAYOUTS)
+@pytest.mark.parametrize("tile", [(256, 256), (128, 128), (64, 64)])
+def test_16bit_k_zero_gives_beta_c(cuda, layout, tile):
+    """K = 0 on every 16-bit engine: no K-tile runs and D = beta * C exactly."""
+    M, N = 264, 200
+    _say(f"case bf16 K=0 {layout} tile={tile}")
+    rng = np.random.default_rng(15)
+    Cb = quantize_bits(colmajor(rng, M, N), "bf16")
+    ta, tb = layout[0] == "T", layout[1] == "T"
+    A = torch.zeros(8 * M, dtype=torch.bfloat16, device=cuda)
+    B = torch.zeros(8 * N, dtype=torch.bfloat16, device=cuda)
+    D = tl.gemm(layout, M, N, 0, A, B, _dev16(Cb, "bf16", cuda), lda=8 if ta else M, ldb=N if tb else 8,
+                alpha=1.0, beta=0.5, tile=tile)
+    torch.cuda.synchronize()
+    want = (0.5 * bits_to_f32(Cb, "bf16")).astype(np.float32)  # a power-of-two scale of bf16 is exact
+    np.testing.assert_array_equal(D.float().cpu().numpy(), want)
+
+
 def test_zero_sized_is_noop(cuda):
     D = torch.full((16,), 3.0, device=cuda)
     tl.gemm("NN", 0, 4, 8, torch.zeros(64, device=cuda), torch.zeros(32, device=cuda), D=D, lda=8, ldd=4)
