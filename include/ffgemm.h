@@ -107,6 +107,19 @@ const char *ffgemm_version(void);
  * (src/gemmMfmaKernelWriter.h:321-405). */
 int ffgemm_query_tile(const ffgemm_problem *p, ffgemm_tile *out);
 
+/* Loads the tuning table that problems run without a tile consult: the
+ * autotuner's best tile per (dtype, layout, M, N, K), one CSV line each,
+ * `dtype,layout,M,N,K,mt_m,mt_n,group_m,waves_m,waves_n[,tflops]`.  It
+ * replaces the table in use.  path NULL: the default, the file named by
+ * FFGEMM_TUNE_DB, else tune_db.csv in the directory above the library's;
+ * "" or a missing file: an empty table.  Without this call the default is
+ * loaded on first use.  The table applies to single problems in the default
+ * compute and output type; an entry that does not fit its problem is passed
+ * over.  *entries (nullable) receives the entry count.  Replaces the
+ * best-of selection of the reference's sweep (test/perf_test.py:58-167),
+ * which picks a configuration by hand from the results sheet. */
+int ffgemm_tune_db_load(const char *path, int *entries);
+
 /* D = alpha*op(A)*op(B) + beta*C on `stream` (hipStream_t, NULL = null stream).
  * Replaces the path GemmMfmaAsmSolution::generateKernel -> StreamHip::Launch
  * (src/gemmMfma.cpp:24-185, ff_gpu_src/ff_hip_runtime.cpp:278-320).

@@ -7,12 +7,15 @@ The reference sweeps padding, LDS buffers, MFMA size, MFMA tiles per wave
 are the macro tile (MT_M x MT_N), the wave tiling (for 256 x 256: eight waves
 of 128 x 64, or four of 128 x 128 -- which also fixes the engine, its LDS
 ring depth and DepthU) and the tile-raster group height.  Results are kept in
-a JSON database keyed by (dtype, layout, M, N, K) and looked up by
-``best_tile``.
+a CSV table keyed by (dtype, layout, M, N, K), one line
+``dtype,layout,M,N,K,mt_m,mt_n,group_m,waves_m,waves_n,tflops`` per problem.
+The C library reads the same table (ffgemm_tune_db_load): a problem run with
+no tile takes its entry.  ``best_tile`` looks an entry up from Python.
 """
 from __future__ import annotations
 
-import json
+import csv
+import ctypes
 import os
 
 import torch
@@ -22,7 +25,11 @@ from .gemm import make_problem, make_tile, run
 
 # (mt_m, mt_n, waves_m, waves_n); waves (0, 0) is the tile's default engine
 TILES = [(256, 256, 2, 4), (256, 256, 2, 2), (256, 128, 0, 0), (128, 256, 0, 0), (128, 128, 0, 0), (64, 64, 0, 0)]
-DB_PATH = os.environ.get("FFGEMM_TUNE_DB", os.path.join(os.path.dirname(os.path.abspath(__file__)), "tune_db.json"))
+DB_PATH = os.environ.get("FFGEMM_TUNE_DB", os.path.join(os.path.dirname(os.path.abspath(__file__)), "tune_db.csv"))
+FIELDS = ["dtype", "layout", "M", "N", "K", "mt_m", "mt_n", "group_m", "waves_m", "waves_n", "tflops"]
+# an entry is recorded only when it beats the library's own choice by this
+# much: run-to-run differences on one box reach ~3 %
+MARGIN = 0.05
 
 
 def _time(fn, steps=10, warmup=3):
@@ -41,52 +48,96 @@ def _time(fn, steps=10, warmup=3):
 
 def load_db(path=None):
     path = path or DB_PATH
+    db = {}
     if os.path.isfile(path):
-        with open(path) as f:
-            return json.load(f)
-    return {}
+        with open(path, newline="") as f:
+            for r in csv.DictReader(f):
+                db[key(int(r["dtype"]), r["layout"], int(r["M"]), int(r["N"]), int(r["K"]))] = {
+                    "tile": [int(r[k]) for k in ("mt_m", "mt_n", "group_m", "waves_m", "waves_n")],
+                    "tflops": float(r["tflops"])}
+    return db
 
 
 def save_db(db, path=None):
+    """Writes the table and has the C library use it from now on."""
     path = path or DB_PATH
     tmp = path + ".tmp"
-    with open(tmp, "w") as f:
-        json.dump(db, f, indent=1, sort_keys=True)
+    with open(tmp, "w", newline="") as f:
+        w = csv.writer(f)
+        w.writerow(FIELDS)
+        for k in sorted(db):
+            dt, lay, mnk = k.split(":")
+            M, N, K = mnk.split("x")
+            e = db[k]
+            w.writerow([dt, lay, M, N, K] + list(e["tile"]) + [round(e["tflops"], 1)])
     os.replace(tmp, path)
+    C.check(C.lib().ffgemm_tune_db_load(path.encode(), None))
 
 
 def key(dtype_code, layout, M, N, K):
     return f"{dtype_code}:{layout}:{M}x{N}x{K}"
 
 
-def tune_gemm(problem: C.Problem, A, B, Cm, D, tiles=TILES, group_ms=(0,), steps=10, verbose=False):
-    """Times every candidate tile for one problem; returns {'best': tile, 'results': [...]}."""
-    results = []
+def tune_gemm(problem: C.Problem, A, B, Cm, D, tiles=TILES, group_ms=(0,), steps=10, rounds=2, verbose=False):
+    """Times every candidate tile for one problem in `rounds` interleaved rounds
+    (each keeps its best time, so no candidate pays alone for the clock ramp);
+    returns {'best': tile, 'results': [...]} fastest first."""
     flop = 2.0 * problem.M * problem.N * problem.K * max(1, problem.batch)
+    cands = []
     for t in tiles:
         wm, wn = (t[2], t[3]) if len(t) > 3 else (0, 0)
         for gm in group_ms:
-            tile = make_tile((t[0], t[1], gm, wm, wn))
+            rec = (t[0], t[1], gm, wm, wn)
             try:
-                ms = _time(lambda: run(problem, A, B, Cm, D, tile))
+                run(problem, A, B, Cm, D, make_tile(rec))
             except C.FFGemmError:
                 continue  # no kernel for this tile / wave tiling with this dtype and layout
-            tf = flop / (ms * 1e-3) / 1e12
-            results.append({"tile": [t[0], t[1], gm, wm, wn], "ms": ms, "tflops": tf})
-            if verbose:
-                print(f"MT{t[0]}x{t[1]} waves={wm}x{wn} gm={gm}: Performance = {tf:.2f} TFLOPs")
-    results.sort(key=lambda r: r["ms"])
+            cands.append(rec)
+    best = {}
+    for _ in range(rounds):
+        for rec in cands:
+            tile = make_tile(rec)
+            best[rec] = min(best.get(rec, float("inf")), _time(lambda: run(problem, A, B, Cm, D, tile), steps=steps))
+    results = sorted(({"tile": list(rec), "ms": ms, "tflops": flop / (ms * 1e-3) / 1e12} for rec, ms in best.items()),
+                     key=lambda r: r["ms"])
+    if verbose:
+        for r in results:
+            t = r["tile"]
+            print(f"MT{t[0]}x{t[1]} waves={t[3]}x{t[4]} gm={t[2]}: Performance = {r['tflops']:.2f} TFLOPs")
     return {"best": tuple(results[0]["tile"]) if results else None, "results": results}
 
 
+def library_default(problem: C.Problem):
+    """The candidate record of the tile the library picks with an empty table
+    (at 256 x 256 with its wave tiling, which the candidates name there)."""
+    C.check(C.lib().ffgemm_tune_db_load(b"", None))
+    try:
+        t = C.Tile()
+        C.check(C.lib().ffgemm_query_tile(ctypes.byref(problem), ctypes.byref(t)))
+    finally:
+        C.check(C.lib().ffgemm_tune_db_load(None, None))
+    if (t.mt_m, t.mt_n) == (256, 256):
+        return (256, 256, 0, t.waves_m, t.waves_n)
+    return (t.mt_m, t.mt_n, 0, 0, 0)
+
+
 def tune_and_record(problem, A, B, Cm, D, db=None, path=None, **kw):
+    """Tunes one problem and records its best tile when that beats the
+    library's own choice by MARGIN; an entry that no longer does is dropped."""
     db = load_db(path) if db is None else db
     r = tune_gemm(problem, A, B, Cm, D, **kw)
-    if r["best"] is not None:
-        lay = ["NN", "NT", "TN", "TT"][problem.layout]
-        db[key(problem.dtype, lay, problem.M, problem.N, problem.K)] = {"tile": list(r["best"]),
-                                                                         "tflops": r["results"][0]["tflops"]}
-        save_db(db, path)
+    if r["best"] is None:
+        return r
+    lay = ["NN", "NT", "TN", "TT"][problem.layout]
+    k = key(problem.dtype, lay, problem.M, problem.N, problem.K)
+    default = library_default(problem)
+    ms_default = next((x["ms"] for x in r["results"] if tuple(x["tile"]) == default), None)
+    r["default"] = default
+    if ms_default is None or r["results"][0]["ms"] < (1.0 - MARGIN) * ms_default:
+        db[k] = {"tile": list(r["best"]), "tflops": r["results"][0]["tflops"]}
+    else:
+        db.pop(k, None)
+    save_db(db, path)
     return r
 
 
@@ -118,3 +169,58 @@ def tune_grouped(problems, As, Bs, Ds, tiles=((256, 256), (256, 128), (128, 256)
         plan.close()
     results = sorted(({"tile": list(t), "ms": ms} for t, ms in best.items()), key=lambda r: r["ms"])
     return {"best": tuple(results[0]["tile"]), "results": results}
+
+
+# The problems the benchmark runs, then common shapes away from the square
+# case: (dtype, layout, M, N, K, output dtype or None)
+TUNE_PROBLEMS = (
+    [(dt, lay, 4096, 4096, 4096, None) for dt in (C.BF16, C.F16) for lay in ("NN", "NT", "TN", "TT")]
+    + [(C.BF16, "NN", 4096, 8192, 4096, None), (C.BF16, "TN", 4096, 8192, 4096, None),   # autograd fwd, dA
+       (C.BF16, "NT", 4096, 4096, 8192, None)]                                            # autograd dB
+    + [(C.F8E4M3, "TN", 4096, 4096, 4096, C.F32), (C.F8E4M3, "TN", 8192, 8192, 8192, C.F32),
+       (C.F32, "NN", 8192, 8192, 8192, None),
+       (C.F16, "TN", 480, 512, 512, None), (C.F32, "NN", 480, 512, 512, None)]
+    + [(C.BF16, "TN", M, N, K, None) for (M, N, K) in
+       ((2048, 2048, 8192), (8192, 1024, 8192), (1024, 8192, 8192), (16384, 16384, 1024), (3072, 3072, 3072),
+        (6144, 6144, 6144), (2048, 8192, 2048), (8192, 2048, 2048), (1024, 1024, 16384), (12288, 4096, 4096))]
+    + [(C.BF16, "NN", M, N, K, None) for (M, N, K) in ((2048, 4096, 4096), (8192, 8192, 2048), (3072, 3072, 3072))])
+
+
+def _operands(dt, lay, M, N, K, out, dev):
+    ta, tb = lay[0] == "T", lay[1] == "T"
+    tdt = {C.BF16: torch.bfloat16, C.F16: torch.float16, C.F32: torch.float32}.get(dt)
+    if dt == C.F8E4M3:  # e4m3 bytes below the NaN encodings; speed does not depend on values
+        A = torch.randint(0, 0x70, (M * K,), dtype=torch.uint8, device=dev)
+        B = torch.randint(0, 0x70, (K * N,), dtype=torch.uint8, device=dev)
+    else:
+        A = (torch.randn(M * K, device=dev) / K ** 0.5).to(tdt)
+        B = (torch.randn(K * N, device=dev) / K ** 0.5).to(tdt)
+    D = torch.empty(M * N, dtype=torch.float32 if (out == C.F32 or dt in (C.F32, C.F8E4M3)) else tdt, device=dev)
+    p = make_problem(lay, dt, M, N, K, K if ta else M, N if tb else K, M, M, dtype_out=out or 0)
+    return p, A, B, D
+
+
+def main(argv=None):
+    import argparse
+    ap = argparse.ArgumentParser(description="Tune the benchmark's problems and record the table.")
+    ap.add_argument("--db", default=None, help="table to update (default: the package's tune_db.csv)")
+    ap.add_argument("--rounds", type=int, default=3)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--group-ms", default="0", help="tile-raster group heights to try, e.g. 0,2,8 (0: default)")
+    a = ap.parse_args(argv)
+    gms = tuple(int(x) for x in a.group_ms.split(","))
+    dev = torch.device("cuda", torch.cuda.current_device())
+    db = load_db(a.db)
+    for (dt, lay, M, N, K, out) in TUNE_PROBLEMS:
+        p, A, B, D = _operands(dt, lay, M, N, K, out, dev)
+        r = tune_and_record(p, A, B, None, D, db=db, path=a.db, rounds=a.rounds, steps=a.steps, group_ms=gms)
+        best = r["results"][0]
+        ms_def = next((x["ms"] for x in r["results"] if tuple(x["tile"]) == r["default"]), float("nan"))
+        rec = "recorded" if key(dt, lay, M, N, K) in db else "library default kept"
+        print(f"dtype {dt} {lay} {M}x{N}x{K}: best {tuple(best['tile'])} {best['ms'] * 1e3:.1f} us, "
+              f"default {r['default']} {ms_def * 1e3:.1f} us: {rec}", flush=True)
+        del A, B, D
+
+
+if __name__ == "__main__":
+    main()

@@ -3,14 +3,20 @@
 // (src/gemmMfma.cpp:24-185), GemmMfmaKernelWriter::checkKernelParameters
 // (src/gemmMfmaKernelWriter.h:321-405) and StreamHip::Launch
 // (ff_gpu_src/ff_hip_runtime.cpp:278-320) for the gemmMfma path.
+#include <dlfcn.h>
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
 #include <cmath>
 #include <cstdarg>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
+#include <fstream>
+#include <map>
 #include <mutex>
+#include <sstream>
+#include <tuple>
 #include <queue>
 #include <string>
 #include <vector>
@@ -85,12 +91,24 @@ int tile_occupancy(int m, int n, int layout) {
 int64_t cdiv(int64_t a, int64_t b) { return (a + b - 1) / b; }
 
 // Largest tile whose grid still fills every CU; else the one with most tiles.
+// The tile with the least estimated time: passes of tiles over the CUs times
+// one tile's time, which is its area over the tile's relative MFMA efficiency
+// (16-bit 4096^3 sweep on MI355X: 256x256 1.0, 256x128 and 128x256 0.82,
+// 128x128 0.70, 64x64 0.51).  The last pass counts whole, so a shape with a
+// few tiles more than the CUs takes the next larger tile rather than a
+// second, nearly empty pass.  Ties go to the larger tile.
 Tile pick_tile(int64_t M, int64_t N, int64_t batch) {
     const int64_t cus = cu_count();
-    for (const Tile& t : kTiles) {
-        if (cdiv(M, t.m) * cdiv(N, t.n) * batch >= cus) return t;
+    static const double eff[] = {1.0, 0.82, 0.82, 0.70, 0.51};  // kTiles order
+    Tile best = kTiles[0];
+    double best_t = 0;
+    for (int i = 0; i < 5; ++i) {
+        const Tile& t = kTiles[i];
+        const int64_t passes = cdiv(cdiv(M, t.m) * cdiv(N, t.n) * batch, cus);
+        const double est = (double)passes * t.m * t.n / eff[i];
+        if (i == 0 || est < best_t) { best = t; best_t = est; }
     }
-    return kTiles[4];
+    return best;
 }
 
 int default_group_m(int64_t tiles_m, int64_t tiles_n) {
@@ -130,9 +148,89 @@ struct Launch {
 
 bool aligned16(const void* p) { return ((uintptr_t)p & 15u) == 0; }
 
+// ---------------------------------------------------------------------------
+// Tuning table: the autotuner's best tile per (dtype, layout, M, N, K), kept
+// as CSV lines `dtype,layout,M,N,K,mt_m,mt_n,group_m,waves_m,waves_n[,tflops]`
+// (layout as NN/NT/TN/TT).  The successor of the best-of selection in
+// test/perf_test.py:58-167: problems run with no tile take the table's tile.
+// ---------------------------------------------------------------------------
+using TuneKey = std::tuple<int, int, int64_t, int64_t, int64_t>;
+std::mutex g_tune_mu;
+std::map<TuneKey, ffgemm_tile> g_tune;
+bool g_tune_loaded = false;
+
+// FFGEMM_TUNE_DB, else tune_db.csv in the directory above the library's own
+// (the package directory when the library sits in <package>/lib).
+std::string default_tune_path() {
+    if (const char* e = getenv("FFGEMM_TUNE_DB")) return e;
+    Dl_info info;
+    if (dladdr((void*)&default_tune_path, &info) && info.dli_fname) {
+        std::string so = info.dli_fname;
+        const size_t a = so.find_last_of('/');
+        if (a != std::string::npos) {
+            const std::string dir = so.substr(0, a);
+            const size_t b = dir.find_last_of('/');
+            if (b != std::string::npos) return dir.substr(0, b) + "/tune_db.csv";
+        }
+    }
+    return "";
+}
+
+// Parses the table at `path` into `out`; returns the number of entries, or -1
+// when the file exists but a line is malformed.  A missing file is an empty table.
+int read_tune_table(const std::string& path, std::map<TuneKey, ffgemm_tile>& out) {
+    out.clear();
+    if (path.empty()) return 0;
+    std::ifstream f(path);
+    if (!f) return 0;
+    std::string line;
+    int n = 0;
+    while (std::getline(f, line)) {
+        if (line.empty() || line[0] == '#' || line.compare(0, 5, "dtype") == 0) continue;
+        std::vector<std::string> v;
+        std::stringstream ss(line);
+        std::string cell;
+        while (std::getline(ss, cell, ',')) v.push_back(cell);
+        if (v.size() < 10) return -1;
+        int layout = -1;
+        const char* names[4] = {"NN", "NT", "TN", "TT"};
+        for (int i = 0; i < 4; ++i)
+            if (v[1] == names[i]) layout = i;
+        char* end = nullptr;
+        if (layout < 0) layout = (int)strtol(v[1].c_str(), &end, 10);
+        long long x[10];
+        for (int i = 0; i < 10; ++i) {
+            if (i == 1) continue;
+            x[i] = strtoll(v[i].c_str(), &end, 10);
+            if (end == v[i].c_str()) return -1;
+        }
+        if (layout < 0 || layout > 3) return -1;
+        ffgemm_tile t;
+        memset(&t, 0, sizeof(t));
+        t.mt_m = (int)x[5]; t.mt_n = (int)x[6]; t.group_m = (int)x[7];
+        t.waves_m = (int)x[8]; t.waves_n = (int)x[9];
+        out[TuneKey((int)x[0], layout, x[2], x[3], x[4])] = t;
+        ++n;
+    }
+    return n;
+}
+
+bool tuned_tile(const ffgemm_problem* p, ffgemm_tile* t) {
+    std::lock_guard<std::mutex> lk(g_tune_mu);
+    if (!g_tune_loaded) {
+        if (read_tune_table(default_tune_path(), g_tune) < 0) g_tune.clear();
+        g_tune_loaded = true;
+    }
+    if (g_tune.empty()) return false;
+    auto it = g_tune.find(TuneKey(p->dtype, p->layout, p->M, p->N, p->K));
+    if (it == g_tune.end()) return false;
+    *t = it->second;
+    return true;
+}
+
 // Checks the problem against what the kernels support and fills `L`.
-int prepare(const ffgemm_problem* p, const void* A, const void* B, const void* C, void* D,
-            const ffgemm_tile* tile, Launch& L) {
+int prepare_with(const ffgemm_problem* p, const void* A, const void* B, const void* C, void* D,
+                 const ffgemm_tile* tile, Launch& L) {
     if (!p) return fail(FFGEMM_ERR, "null problem");
     if (p->layout < 0 || p->layout > 3) return fail(FFGEMM_ERR, "bad layout %d", p->layout);
     const int64_t M = p->M, N = p->N, K = p->K;
@@ -242,6 +340,19 @@ int prepare(const ffgemm_problem* p, const void* A, const void* B, const void* C
     return FFGEMM_SUCCESS;
 }
 
+// prepare_with, where a problem given no tile first tries the tuning table's
+// tile for it (single problems in the default compute and output type); an
+// entry that does not fit the problem is passed over.
+int prepare(const ffgemm_problem* p, const void* A, const void* B, const void* C, void* D,
+            const ffgemm_tile* tile, Launch& L) {
+    ffgemm_tile tuned;
+    if (p && (!tile || !tile->mt_m) && p->batch <= 1 && p->compute == FFGEMM_COMPUTE_DEFAULT &&
+        (p->dtype_out == 0 || p->dtype_out == p->dtype || p->dtype == FFGEMM_F8E4M3) && tuned_tile(p, &tuned) &&
+        prepare_with(p, A, B, C, D, &tuned, L) == FFGEMM_SUCCESS)
+        return FFGEMM_SUCCESS;
+    return prepare_with(p, A, B, C, D, tile, L);
+}
+
 int launch(const Launch& L, hipStream_t s) {
     int r = -1;
     switch (L.mode) {
@@ -305,6 +416,18 @@ const char* ffgemm_status_string(int status) {
 }
 
 const char* ffgemm_last_error(void) { return g_last_error.c_str(); }
+
+int ffgemm_tune_db_load(const char* path, int* entries) {
+    std::map<TuneKey, ffgemm_tile> t;
+    const std::string where = path ? std::string(path) : default_tune_path();
+    const int n = read_tune_table(where, t);
+    if (n < 0) return fail(FFGEMM_ERR, "malformed tuning table %s", where.c_str());
+    std::lock_guard<std::mutex> lk(g_tune_mu);
+    g_tune.swap(t);
+    g_tune_loaded = true;
+    if (entries) *entries = n;
+    return FFGEMM_SUCCESS;
+}
 
 const char* ffgemm_version(void) { return "ffgemm 0.1 gfx950"; }
 

@@ -7,6 +7,10 @@ ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 if ROOT not in sys.path:
     sys.path.insert(0, ROOT)
 
+# Tests run with an empty tuning table, so every problem without a tile gets
+# the library's own choice; tests of the table load one explicitly.
+os.environ["FFGEMM_TUNE_DB"] = ""
+
 
 def pytest_configure(config):
     config.addinivalue_line("markers", "gpu: needs an MI355X (gfx950) GPU")

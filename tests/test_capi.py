@@ -52,6 +52,21 @@ def test_query_tile_accepts_supported(layout, dtype):
     assert t.mfma_mn == 32
 
 
+@pytest.mark.parametrize("shape,tile", [
+    ((4096, 4096, 4096), (256, 256)),
+    ((3072, 3072, 3072), (256, 256)),    # 144 tiles in one pass beat 288 in two (256 x 128)
+    ((2048, 2048, 8192), (128, 128)),
+    ((8192, 1024, 8192), (256, 128)),
+    ((1024, 1024, 16384), (64, 64)),
+    ((6144, 6144, 6144), (256, 256)),
+])
+def test_default_tile_is_the_measured_best(shape, tile):
+    """With no tile and an empty table the library estimates passes x tile
+    time; these shapes' tiles are the autotuner's measured winners (bf16 TN)."""
+    t = tl.query_tile(_p("TN", C.BF16, *shape))
+    assert (t.mt_m, t.mt_n) == tile
+
+
 def test_query_tile_small_shapes_pick_small_tiles():
     t = tl.query_tile(_p("NN", C.BF16, 480, 512, 512))
     assert (t.mt_m, t.mt_n) == (64, 64)
@@ -189,14 +204,75 @@ def test_wave_tiling_knob_rejects(layout, dtype, tile, waves):
 
 
 def test_tuned_tile_lookup(tmp_path, monkeypatch):
-    """tile='tuned' reads the autotuner database (a missing entry means the library's choice)."""
+    """tile='tuned' reads the autotuner table (a missing entry means the library's choice)."""
     import importlib
     from tensilelite_amd import autotune
     G = importlib.import_module("tensilelite_amd.gemm")   # the module (the package re-exports gemm())
-    monkeypatch.setattr(autotune, "DB_PATH", str(tmp_path / "db.json"))
-    autotune.save_db({autotune.key(C.BF16, "TN", 4096, 4096, 4096): {"tile": [256, 256, 0, 2, 2], "tflops": 1.0}})
-    p = _p("TN", C.BF16, 4096, 4096, 4096)
-    assert list(G._tuned(p)) == [256, 256, 0, 2, 2]
-    t = tl.query_tile(p, G.make_tile(G._tuned(p)))
-    assert (t.waves_m, t.waves_n) == (2, 2)
-    assert G._tuned(_p("TN", C.BF16, 1024, 1024, 1024)) is None
+    monkeypatch.setattr(autotune, "DB_PATH", str(tmp_path / "db.csv"))
+    try:
+        autotune.save_db({autotune.key(C.BF16, "TN", 4096, 4096, 4096): {"tile": [256, 256, 0, 2, 2], "tflops": 1.0}})
+        p = _p("TN", C.BF16, 4096, 4096, 4096)
+        assert list(G._tuned(p)) == [256, 256, 0, 2, 2]
+        t = tl.query_tile(p, G.make_tile(G._tuned(p)))
+        assert (t.waves_m, t.waves_n) == (2, 2)
+        assert G._tuned(_p("TN", C.BF16, 1024, 1024, 1024)) is None
+    finally:
+        C.check(C.lib().ffgemm_tune_db_load(b"", None))
+
+
+def _load_table(path):
+    n = ctypes.c_int32(-1)
+    st = C.lib().ffgemm_tune_db_load(str(path).encode(), ctypes.byref(n))
+    return st, n.value
+
+
+def test_tune_table_consulted_by_the_library(tmp_path):
+    """Problems given no tile take the table's entry; an entry that does not
+    fit its problem is passed over; a malformed table is refused."""
+    f = tmp_path / "t.csv"
+    f.write_text("dtype,layout,M,N,K,mt_m,mt_n,group_m,waves_m,waves_n,tflops\n"
+                 f"{C.BF16},TN,4096,4096,4096,256,256,0,2,2,1000.0\n"
+                 f"{C.F32},NN,4096,4096,4096,256,256,0,2,4,100.0\n"      # no eight-wave fp32 kernel
+                 f"{C.BF16},NN,480,512,512,128,128,4,0,0,50.0\n")
+    try:
+        assert _load_table(f) == (C.SUCCESS, 3)
+        t = tl.query_tile(_p("TN", C.BF16, 4096, 4096, 4096))          # the table's four-wave engine
+        assert (t.mt_m, t.mt_n, t.waves_m, t.waves_n) == (256, 256, 2, 2)
+        t = tl.query_tile(_p("NN", C.F32, 4096, 4096, 4096))           # passed over: the default
+        assert (t.mt_m, t.mt_n) == (256, 256) and t.waves_n == 2
+        t = tl.query_tile(_p("NN", C.BF16, 480, 512, 512))
+        assert (t.mt_m, t.mt_n, t.group_m) == (128, 128, 4)
+        t = tl.query_tile(_p("TN", C.BF16, 4096, 4096, 1024))          # not in the table
+        assert (t.mt_m, t.mt_n, t.waves_n) == (256, 256, 4)
+        bad = tmp_path / "bad.csv"
+        bad.write_text("1,TN,64,64\n")
+        assert _load_table(bad)[0] == C.ERR
+        assert _load_table(tmp_path / "missing.csv") == (C.SUCCESS, 0)
+    finally:
+        C.check(C.lib().ffgemm_tune_db_load(b"", None))
+    t = tl.query_tile(_p("NN", C.BF16, 480, 512, 512))
+    assert (t.mt_m, t.mt_n) == (64, 64)
+
+
+def test_committed_tune_table_is_valid():
+    """Every entry of the committed table is accepted as is for its problem."""
+    from tensilelite_amd import autotune
+    path = os.path.join(os.path.dirname(os.path.abspath(autotune.__file__)), "tune_db.csv")
+    db = autotune.load_db(path)
+    assert os.path.isfile(path)
+    try:
+        st, n = _load_table(path)
+        assert st == C.SUCCESS and n == len(db)
+        for k, e in db.items():
+            dt, lay, mnk = k.split(":")
+            M, N, K = (int(x) for x in mnk.split("x"))
+            p = _p(lay, int(dt), M, N, K)
+            t = tl.query_tile(p)
+            mt_m, mt_n, gm, wm, wn = e["tile"]
+            assert (t.mt_m, t.mt_n) == (mt_m, mt_n), k
+            if wm:
+                assert (t.waves_m, t.waves_n) == (wm, wn), k
+            if gm:
+                assert t.group_m == gm, k
+    finally:
+        C.check(C.lib().ffgemm_tune_db_load(b"", None))
