@@ -153,6 +153,20 @@ int ffgemm_run_tn_kernarg(void *D, const void *C, const void *A, const void *B, 
                           uint32_t N, uint32_t K, uint32_t batch, int dtype,
                           const ffgemm_tile *tile, void *stream);
 
+/* The reference's Tensile-layout kernels (-t 1: Cijk_Alik_Bljk_{SB,HBH,BBH},
+ * src/gemmMfma.cpp:72-102) take a 148-byte argument buffer: sizeC, sizeA,
+ * sizeB (u64, elements), D, C, A, B, alpha, beta (32-bit slots), strideD0,
+ * strideD1, StrideC0, StrideC1, StrideA0, StrideA1, StrideB0, StrideB1,
+ * SizesFree0..2 (M, N, batch), SizesSum0 (K), then eight grid-mapping values
+ * and a pad.  This entry runs the TN problem such a buffer describes, for a
+ * caller that builds it: alpha/beta are fp16 in the low half of their slots
+ * for F16 (as the reference's HBH kernels read them,
+ * src/gemmMfmaKernelWriter.h:1845-1854) and fp32 otherwise; nonzero sizes
+ * must cover what the problem addresses; the grid-mapping values are ignored
+ * (the library maps its own grid).  dtype: FFGEMM_F32, F16 or BF16. */
+int ffgemm_run_tensile_args(const void *kernarg, size_t kernarg_bytes, int dtype, const ffgemm_tile *tile,
+                            void *stream);
+
 /* FP8 e4m3 per-tensor quantisation: scale = 448 / max|x| (fp32, computed on
  * the device into *scale_out), q = rne_sat(x * scale).  x is an M x K (or
  * K x N) fp32 matrix given in `layout_of_x` terms: for operand 'A' with

@@ -545,6 +545,61 @@ int ffgemm_run_tn_kernarg(void* D, const void* C, const void* A, const void* B, 
     return ffgemm_run(&p, A, B, C, D, tile, stream);
 }
 
+int ffgemm_run_tensile_args(const void* kernarg, size_t kernarg_bytes, int dtype, const ffgemm_tile* tile,
+                            void* stream) {
+    // byte offsets of the reference's Tensile-layout argument buffer
+    // (src/gemmMfma.cpp:72-102; its kernels' metadata lists the same)
+    enum : size_t {
+        kSizeC = 0, kSizeA = 8, kSizeB = 16, kD = 24, kC = 32, kA = 40, kB = 48, kAlpha = 56, kBeta = 60,
+        kStrideD0 = 64, kStrideD1 = 68, kStrideC0 = 72, kStrideC1 = 76, kStrideA0 = 80, kStrideA1 = 84,
+        kStrideB0 = 88, kStrideB1 = 92, kFree0 = 96, kFree1 = 100, kFree2 = 104, kSum0 = 108, kBytes = 148
+    };
+    if (!kernarg || kernarg_bytes < kBytes)
+        return fail(FFGEMM_ERR, "Tensile argument buffer of %zu bytes; %d expected", kernarg_bytes, (int)kBytes);
+    if (dtype < FFGEMM_F32 || dtype > FFGEMM_BF16) return fail(FFGEMM_ERR, "data type %d not in {1,2,3}", dtype);
+    const char* k = (const char*)kernarg;
+    auto u32 = [&](size_t off) { uint32_t v; memcpy(&v, k + off, 4); return v; };
+    auto u64 = [&](size_t off) { uint64_t v; memcpy(&v, k + off, 8); return v; };
+    auto ptr = [&](size_t off) { void* v; memcpy(&v, k + off, sizeof(v)); return v; };
+    // alpha / beta: 32-bit slots; the half-precision (HBH) kernels take fp16 in
+    // the low half (src/gemmMfmaKernelWriter.h:1845-1854), the others fp32
+    auto scalar = [&](size_t off) {
+        const uint32_t bits = u32(off);
+        if (dtype == FFGEMM_F16) {
+            const uint16_t h = (uint16_t)(bits & 0xffffu);
+            _Float16 x;
+            memcpy(&x, &h, 2);
+            return (float)x;
+        }
+        float f;
+        memcpy(&f, &bits, 4);
+        return f;
+    };
+    ffgemm_problem p;
+    memset(&p, 0, sizeof(p));
+    p.layout = FFGEMM_TN; p.dtype = dtype; p.dtype_out = dtype;
+    p.M = u32(kFree0); p.N = u32(kFree1); p.K = u32(kSum0);
+    p.batch = u32(kFree2) ? u32(kFree2) : 1;
+    p.ldd = u32(kStrideD0); p.bsD = u32(kStrideD1);
+    p.ldc = u32(kStrideC0); p.bsC = u32(kStrideC1);
+    p.lda = u32(kStrideA0); p.bsA = u32(kStrideA1);
+    p.ldb = u32(kStrideB0); p.bsB = u32(kStrideB1);
+    p.alpha = scalar(kAlpha); p.beta = scalar(kBeta);
+    // sizeA/B/C (elements; 0: not given) must cover what the problem addresses;
+    // sizeC is the C/D tensor: D always, C when beta != 0
+    if (p.M && p.N) {
+        const int64_t b = p.batch - 1;
+        const int64_t needA = p.K ? b * p.bsA + (p.M - 1) * p.lda + p.K : 0;
+        const int64_t needB = p.K ? b * p.bsB + (p.N - 1) * p.ldb + p.K : 0;
+        int64_t needC = b * p.bsD + (p.N - 1) * p.ldd + p.M;
+        if (p.beta != 0.0f) needC = std::max<int64_t>(needC, b * p.bsC + (p.N - 1) * p.ldc + p.M);
+        const int64_t sA = (int64_t)u64(kSizeA), sB = (int64_t)u64(kSizeB), sC = (int64_t)u64(kSizeC);
+        if ((sA && sA < needA) || (sB && sB < needB) || (sC && sC < needC))
+            return fail(FFGEMM_ERR, "sizeA/sizeB/sizeC smaller than the problem addresses");
+    }
+    return ffgemm_run(&p, ptr(kA), ptr(kB), ptr(kC), ptr(kD), tile, stream);
+}
+
 size_t ffgemm_fp8_workspace_size(void) { return 256; }
 
 int ffgemm_fp8_quantize(const float* x, int64_t rows, int64_t k, int64_t ldx, int trans_x, uint8_t* q,

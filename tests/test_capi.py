@@ -122,6 +122,28 @@ def test_kernarg_entry_rejects_bad_dtype_without_launch():
     assert b"data type" in C.lib().ffgemm_last_error()
 
 
+def tensile_args(D, C_, A, B, alpha_bits, beta_bits, M, N, K, batch, ldd, bsD, ldc, bsC, lda, bsA, ldb, bsB,
+                 sizes=(0, 0, 0)):
+    """The reference's 148-byte Tensile-layout argument buffer (src/gemmMfma.cpp:72-102)."""
+    import struct
+    buf = struct.pack("<3Q4Q2I8I4I9I", sizes[0], sizes[1], sizes[2], D, C_, A, B, alpha_bits, beta_bits,
+                      ldd, bsD, ldc, bsC, lda, bsA, ldb, bsB, M, N, batch, K, *([0] * 9))
+    assert len(buf) == 148
+    return buf
+
+
+def test_tensile_args_entry_rejects_without_launch():
+    lib = C.lib()
+    buf = tensile_args(16, 16, 16, 16, 0x3F800000, 0, 64, 64, 64, 1, 64, 4096, 64, 4096, 64, 4096, 64, 4096)
+    assert lib.ffgemm_run_tensile_args(buf, 100, C.BF16, None, None) == C.ERR          # short buffer
+    assert b"148" in lib.ffgemm_last_error()
+    assert lib.ffgemm_run_tensile_args(buf, 148, 7, None, None) == C.ERR               # data type
+    small = tensile_args(16, 16, 16, 16, 0x3F800000, 0, 64, 64, 64, 1, 64, 4096, 64, 4096, 64, 4096, 64, 4096,
+                         sizes=(4096, 64 * 63, 4096))                                 # sizeA one row short
+    assert lib.ffgemm_run_tensile_args(small, 148, C.BF16, None, None) == C.ERR
+    assert b"sizeA" in lib.ffgemm_last_error()
+
+
 def test_grouped_plan_rejects_empty_group():
     h = ctypes.c_void_p()
     st = C.lib().ffgemm_grouped_plan_create(ctypes.byref(h), 0, None, None, None, None, None, None)

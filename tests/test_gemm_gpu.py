@@ -313,6 +313,50 @@ def test_reference_kat_tn_kernarg(cuda, dtype):
         assert bad == 0, worst
 
 
+@pytest.mark.parametrize("name", ["bf16", "f16", "f32"])
+def test_tensile_args_entry(cuda, name):
+    """A Tensile-style caller's 148-byte argument buffer (BBH / HBH / SB), batch 2
+    with padded batch strides; alpha / beta as those kernels take them."""
+    from tests.test_capi import tensile_args
+    M, N, K, batch = 200, 136, 96, 2
+    _say(f"case Tensile argument buffer {name}")
+    rng = np.random.default_rng(16)
+    lda = ldb = K + 8
+    ldc = ldd = M + 8
+    bsA, bsB, bsC = lda * M + 64, ldb * N + 32, ldc * N + 16
+    A, B, Cm = (rng.uniform(-1.0, 1.0, n).astype(np.float32) for n in (bsA * batch, bsB * batch, bsC * batch))
+    alpha, beta = 0.75, -0.5
+    if name == "f32":
+        Ad, Bd, Cd = (torch.from_numpy(np.ascontiguousarray(x)).to(cuda) for x in (A, B, Cm))
+        Ar, Br, Cr = A, B, Cm
+        ab = [int(np.float32(v).view(np.uint32)) for v in (alpha, beta)]
+        dt = C.F32
+    else:
+        bits = [quantize_bits(np.ascontiguousarray(x), name) for x in (A, B, Cm)]
+        Ad, Bd, Cd = (_dev16(b_, name, cuda) for b_ in bits)
+        Ar, Br, Cr = (bits_to_f32(b_, name) for b_ in bits)
+        if name == "f16":   # HBH: fp16 bits in the low half of each slot
+            ab = [int(np.float16(v).view(np.uint16)) for v in (alpha, beta)]
+        else:
+            ab = [int(np.float32(v).view(np.uint32)) for v in (alpha, beta)]
+        dt = C.F16 if name == "f16" else C.BF16
+    D = torch.full((bsC * batch,), 7.0, dtype=Cd.dtype, device=cuda)
+    buf = tensile_args(D.data_ptr(), Cd.data_ptr(), Ad.data_ptr(), Bd.data_ptr(), ab[0], ab[1], M, N, K, batch,
+                       ldd, bsC, ldc, bsC, lda, bsA, ldb, bsB, sizes=(bsC * batch, bsA * batch, bsB * batch))
+    C.check(C.lib().ffgemm_run_tensile_args(buf, len(buf), dt, None, None))
+    torch.cuda.synchronize()
+    ref = _f64_ref("TN", M, N, K, Ar, lda, Br, ldb, alpha, beta, Cr, ldc, ldd, batch, bsA, bsB, bsC, bsC)
+    got = D.float().cpu().numpy()
+    for b in range(batch):
+        r = ref[b * bsC: b * bsC + ldd * N].reshape(N, ldd)[:, :M]
+        g = got[b * bsC: b * bsC + ldd * N].reshape(N, ldd)[:, :M]
+        if name == "f32":
+            assert rel_fro(g, r) <= 1e-5
+        else:
+            bad, worst = elem_ok(g, r)
+            assert bad == 0, (b, worst)
+
+
 def test_bf16_truncate_matches_reference_store(cuda):
     """bf16 D rounded by truncation, as the reference's store
     (src/gemmMfmaKernelWriter.h:1661-1664).  Integer data keeps every fp32 sum
