@@ -340,6 +340,23 @@ int prepare_with(const ffgemm_problem* p, const void* A, const void* B, const vo
     return FFGEMM_SUCCESS;
 }
 
+// An M/N-contiguous operand is addressed whole, all K rows of its ld, with
+// 31-bit buffer offsets.  When that exceeds 2 GiB the problem runs in K chunks
+// (a multiple of 64 that fits), accumulating into D; 0 when no chunking is
+// needed (or possible: then prepare reports the size).
+int64_t mn_k_chunk(const ffgemm_problem* p) {
+    if (!p || p->K <= 0 || p->M <= 0 || p->N <= 0) return 0;
+    const bool ta = p->layout >= FFGEMM_TN, tb = (p->layout & 1) != 0;
+    const int es = p->dtype == FFGEMM_F32 ? 4 : p->dtype == FFGEMM_F8E4M3 ? 1 : 2;
+    int64_t ld = 0;
+    if (!ta) ld = std::max<int64_t>(ld, p->lda);  // A M-contiguous: K rows of lda
+    if (tb) ld = std::max<int64_t>(ld, p->ldb);   // B N-contiguous: K rows of ldb
+    const int64_t lim = (int64_t)1 << 31;
+    if (ld <= 0 || (p->K + 256) * ld * es < lim) return 0;
+    const int64_t kc = ((lim - 1) / (ld * es) - 256) / 64 * 64;
+    return kc >= 64 ? kc : 0;
+}
+
 // prepare_with, where a problem given no tile first tries the tuning table's
 // tile for it (single problems in the default compute and output type); an
 // entry that does not fit the problem is passed over.
@@ -438,8 +455,15 @@ int ffgemm_query_tile(const ffgemm_problem* p, ffgemm_tile* out) {
         if (out) { memset(out, 0, sizeof(*out)); out->mt_m = 64; out->mt_n = 64; out->depth_u = 16; }
         return FFGEMM_SUCCESS;
     }
-    // validation with dummy aligned pointers; nothing is dereferenced
+    // validation with dummy aligned pointers; nothing is dereferenced.  A
+    // problem that runs in K chunks is checked through its first chunk.
     static const int64_t dummy[4] = {0, 0, 0, 0};
+    ffgemm_problem q;
+    if (const int64_t kc = mn_k_chunk(p)) {
+        q = *p;
+        q.K = kc;
+        p = &q;
+    }
     Launch L;
     int r = prepare(p, dummy, dummy, dummy, (void*)dummy, out && out->mt_m ? out : nullptr, L);
     if (r != FFGEMM_SUCCESS) return r;
@@ -457,6 +481,25 @@ int ffgemm_query_tile(const ffgemm_problem* p, ffgemm_tile* out) {
 int ffgemm_run(const ffgemm_problem* p, const void* A, const void* B, const void* C, void* D,
                const ffgemm_tile* tile, void* stream) {
     if (p && p->dtype == FFGEMM_F64) return run_f64(p, A, B, C, D, (hipStream_t)stream);
+    if (const int64_t kc = mn_k_chunk(p)) {
+        // K chunks: the first applies the caller's beta * C, the rest add into D
+        // (C = D, beta = 1; each element is read and written by one thread)
+        const bool ta = p->layout >= FFGEMM_TN, tb = (p->layout & 1) != 0;
+        const int es = p->dtype == FFGEMM_F32 ? 4 : 2;
+        for (int64_t k0 = 0; k0 < p->K; k0 += kc) {
+            ffgemm_problem q = *p;
+            q.K = std::min<int64_t>(kc, p->K - k0);
+            const void* Aq = (const char*)A + (ta ? k0 : k0 * p->lda) * es;
+            const void* Bq = (const char*)B + (tb ? k0 * p->ldb : k0) * es;
+            const void* Cq = C;
+            if (k0 > 0) { q.beta = 1.0f; Cq = D; q.ldc = p->ldd; q.bsC = p->bsD; }
+            Launch L;
+            int r = prepare(&q, Aq, Bq, Cq, D, tile, L);
+            if (r == FFGEMM_SUCCESS) r = launch(L, (hipStream_t)stream);
+            if (r != FFGEMM_SUCCESS) return r;
+        }
+        return FFGEMM_SUCCESS;
+    }
     Launch L;
     int r = prepare(p, A, B, C, D, tile, L);
     if (r != FFGEMM_SUCCESS) return r;
@@ -500,7 +543,7 @@ int ffgemm_workspace_size(const ffgemm_problem* p, size_t* bytes) {
 int ffgemm_run_ws(const ffgemm_problem* p, const void* A, const void* B, const void* C, void* D,
                   const ffgemm_tile* tile, void* workspace, size_t workspace_bytes, void* stream) {
     if (!p) return fail(FFGEMM_ERR, "null problem");
-    if (!x3_planes_apply(p) || !workspace || workspace_bytes < (size_t)x3_ws_bytes(p))
+    if (!x3_planes_apply(p) || !workspace || workspace_bytes < (size_t)x3_ws_bytes(p) || mn_k_chunk(p))
         return ffgemm_run(p, A, B, C, D, tile, stream);
     if ((uintptr_t)workspace % 256) return fail(FFGEMM_ERR, "workspace must be 256-byte aligned");
     Launch L;

@@ -94,16 +94,19 @@ def test_query_tile_rejects_with_rtn_err(bad, msg):
 
 def test_operand_size_limits():
     """Buffer offsets are 31-bit: a K-contiguous operand is addressed from each
-    tile's first row, so it may exceed 2 GiB; an M/N-contiguous one may not."""
+    tile's first row, so it may exceed 2 GiB; a larger M/N-contiguous one runs
+    in K chunks; only an ld too long for 320 rows in 2 GiB is refused."""
     M, N, K = 140000, 256, 8192                   # A: 140000 rows of 8192 bf16 = 2.29 GB
     t = tl.query_tile(_p("TN", C.BF16, M, N, K))  # A and B K-contiguous
     assert t.mt_m > 0
-    with pytest.raises(tl.FFGemmError) as e:      # NN: A is M-contiguous, K rows of lda = M
-        tl.query_tile(_p("NN", C.BF16, M, N, K))
-    assert e.value.status == C.ERR and "2 GiB" in str(e.value)
+    t = tl.query_tile(_p("NN", C.BF16, M, N, K))  # NN: A M-contiguous, 8192 rows of lda = M: K chunks
+    assert t.mt_m > 0
     Kw = 4194304                                  # 257 rows of 2^22 bf16 exceed 2 GiB
-    with pytest.raises(tl.FFGemmError):
+    with pytest.raises(tl.FFGemmError) as e:
         tl.query_tile(_p("TN", C.BF16, 512, 512, Kw))
+    assert e.value.status == C.ERR and "2 GiB" in str(e.value)
+    with pytest.raises(tl.FFGemmError):           # M-contiguous: 320 rows of lda = 2^22 exceed 2 GiB
+        tl.query_tile(_p("NN", C.BF16, Kw, 64, 1024))
     # bf16x3 planes are K-contiguous too: a workspace is offered past 2 GiB of planes
     x3 = tl.make_problem("TN", C.F32, M, N, K, K, K, M, M, compute=C.COMPUTE_BF16X3)
     assert tl.workspace_size(x3) == ((M * 3 * K * 2 + 255) // 256 + (N * 3 * K * 2 + 255) // 256) * 256

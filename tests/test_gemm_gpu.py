@@ -206,6 +206,34 @@ def test_tn_operand_past_2gib(cuda):
     torch.cuda.empty_cache()
 
 
+def test_nn_m_contiguous_operand_past_2gib(cuda):
+    """bf16 NN with an M-contiguous A of K = 8192 rows x lda = 140000 (2.29 GB):
+    the library runs it in two K chunks accumulating into D (beta * C applied
+    once); sampled rows on both sides of the 2 GiB mark are checked."""
+    M, N, K = 140000, 256, 8192
+    _say(f"case NN M-contiguous operand past 2 GiB M={M} N={N} K={K}")
+    g = torch.Generator(device=cuda).manual_seed(17)
+    A = torch.empty(M * K, dtype=torch.bfloat16, device=cuda)
+    for k0 in range(0, K, 1024):  # fill in slices to bound the fp32 temporary
+        A[k0 * M:(k0 + 1024) * M] = (torch.randn(1024 * M, generator=g, device=cuda) / K ** 0.5).to(torch.bfloat16)
+    B = (torch.randn(K * N, generator=g, device=cuda) / K ** 0.5).to(torch.bfloat16)
+    Cm = torch.randn(M * N, generator=g, device=cuda).to(torch.bfloat16)
+    D = tl.gemm("NN", M, N, K, A, B, Cm, alpha=1.0, beta=0.5)
+    torch.cuda.synchronize()
+    rows = [0, 1, 255, 256, 65535, 65536, 76695, 76696, 139743, 139999]
+    rows += torch.randint(0, M, (22,), generator=torch.Generator().manual_seed(18)).tolist()
+    rows = sorted(set(rows))
+    Ar = A.view(K, M)[:, rows].float().cpu().numpy().astype(np.float64).ravel()   # R-contiguous, K columns
+    Bh = B.float().cpu().numpy().astype(np.float64)
+    Cr = Cm.view(N, M)[:, rows].float().cpu().numpy().astype(np.float64).ravel()
+    ref = oracle.gemm_f64("NN", len(rows), N, K, 1.0, Ar, len(rows), Bh, K, 0.5, Cr, len(rows)).reshape(N, len(rows))
+    got = D.view(N, M)[:, rows].float().cpu().numpy()
+    bad, worst = elem_ok(got, ref)
+    assert bad == 0, worst
+    del A, B, Cm, D
+    torch.cuda.empty_cache()
+
+
 def test_k_zero_and_alpha_zero(cuda):
     M, N = 96, 80
     _say("case K=0 / alpha=0")
