@@ -342,6 +342,107 @@ __device__ inline void store16(const TileArgs& a, const char* Cb, char* Db, int 
     }
 }
 
+// Epilogue of a wave's NF accumulator fragments held in VGPRs (the eight-wave
+// engine's 128 registers); fragment f covers rows m[f] .. m[f]+15 of column
+// n[f] (lane layout as in store16).  With beta != 0 the C slices of up to NB
+// fragments are requested before the first one is used, so their memory
+// latencies overlap instead of adding up; a fragment not wholly inside D reads
+// C's first 16 rows instead (valid when M >= 16) and takes store16's
+// per-element path.  The beta and rounding choices are template parameters
+// behind one uniform branch each.  (The four-wave engines keep 256
+// accumulator registers in AGPRs and store fragment by fragment with store16.)
+template <int OUT, int NF, int NB, bool USE_C, bool TRUNC>
+__device__ inline void store_frags_as(const TileArgs& a, const char* Cb, char* Db, const int* m, const int* n,
+                                      const f32x16_t* acc, float alpha, float beta) {
+    constexpr int OS = (OUT == kOutF32) ? 4 : 2;
+    constexpr int CV = OS;  // 16-byte C vectors per fragment: 16 values of OS bytes
+    static_assert(NF % NB == 0 || NF < NB, "fragment batches");
+    constexpr int B = NF < NB ? NF : NB;
+#pragma unroll
+    for (int f0 = 0; f0 < NF; f0 += B) {
+        i32x4_t cv[B][CV];
+        if constexpr (USE_C) {
+            if (a.M >= 16) {
+#pragma unroll
+                for (int f = 0; f < B; ++f) {
+                    const bool full = n[f0 + f] < a.N && m[f0 + f] + 16 <= a.M;
+                    const char* cp = full ? Cb + ((int64_t)n[f0 + f] * a.ldc + m[f0 + f]) * OS : Cb;
+#pragma unroll
+                    for (int q = 0; q < CV; ++q) cv[f][q] = *(const i32x4_t*)(cp + 16 * q);
+                }
+            }
+        }
+#pragma unroll
+        for (int f = 0; f < B; ++f) {
+            const int mm = m[f0 + f], nn = n[f0 + f];
+            if (nn >= a.N || mm >= a.M) continue;
+            const f32x16_t& x = acc[f0 + f];
+            if (mm + 16 > a.M) {  // rows past M: element by element
+                float v[16];
+#pragma unroll
+                for (int e = 0; e < 16; ++e) v[e] = x[e];
+                store16<OUT>(a, Cb, Db, mm, nn, v, alpha, beta);
+                continue;
+            }
+            float r[16];
+            if constexpr (USE_C) {
+                float c[16];
+#pragma unroll
+                for (int q = 0; q < CV; ++q) {
+#pragma unroll
+                    for (int e = 0; e < 4; ++e) {
+                        const uint32_t u = (uint32_t)cv[f][q][e];
+                        if constexpr (OUT == kOutF32) {
+                            c[4 * q + e] = __builtin_bit_cast(float, u);
+                        } else {
+                            c[8 * q + 2 * e] = (OUT == kOutBF16) ? bf16lo(u) : f16lo(u);
+                            c[8 * q + 2 * e + 1] = (OUT == kOutBF16) ? bf16hi(u) : f16hi(u);
+                        }
+                    }
+                }
+#pragma unroll
+                for (int e = 0; e < 16; ++e) r[e] = __builtin_fmaf(alpha, x[e], beta * c[e]);
+            } else {
+#pragma unroll
+                for (int e = 0; e < 16; ++e) r[e] = alpha * x[e];
+            }
+            char* dp = Db + ((int64_t)nn * a.ldd + mm) * OS;
+            if constexpr (OUT == kOutF32) {
+#pragma unroll
+                for (int q = 0; q < 4; ++q)
+                    *(f32x4_t*)(dp + 16 * q) = f32x4_t{r[4 * q], r[4 * q + 1], r[4 * q + 2], r[4 * q + 3]};
+            } else {
+#pragma unroll
+                for (int q = 0; q < 2; ++q) {
+                    uint32_t w[4];
+#pragma unroll
+                    for (int e = 0; e < 4; ++e) {
+                        const float lo = r[8 * q + 2 * e], hi = r[8 * q + 2 * e + 1];
+                        if constexpr (OUT == kOutBF16) w[e] = TRUNC ? pack_bf16x2_trunc(lo, hi) : pack_bf16x2_rne(lo, hi);
+                        else w[e] = pack_f16x2(lo, hi);
+                    }
+                    *(i32x4_t*)(dp + 16 * q) = i32x4_t{(int)w[0], (int)w[1], (int)w[2], (int)w[3]};
+                }
+            }
+        }
+    }
+}
+
+// NB: fragments whose C is in flight together; each holds 2 (16-bit C) or 4
+// (fp32 C) 16-byte vectors, so engines size it to their free registers.
+template <int OUT, int NF, int NB>
+__device__ inline void store_frags(const TileArgs& a, const char* Cb, char* Db, const int* m, const int* n,
+                                   const f32x16_t* acc, float alpha, float beta) {
+    const bool trunc = OUT == kOutBF16 && (a.flags & kFlagTrunc) != 0;
+    if (a.flags & kFlagBeta) {
+        if (trunc) store_frags_as<OUT, NF, NB, true, true>(a, Cb, Db, m, n, acc, alpha, beta);
+        else store_frags_as<OUT, NF, NB, true, false>(a, Cb, Db, m, n, acc, alpha, beta);
+    } else {
+        if (trunc) store_frags_as<OUT, NF, NB, false, true>(a, Cb, Db, m, n, acc, alpha, beta);
+        else store_frags_as<OUT, NF, NB, false, false>(a, Cb, Db, m, n, acc, alpha, beta);
+    }
+}
+
 // Extent in bytes of one batch's operand as a buffer: the last row ends at
 // `tail` elements, rounded up to the 16-byte vector that holds it (a partly
 // out-of-range vector would read as all zeros; an aligned 16-byte vector never

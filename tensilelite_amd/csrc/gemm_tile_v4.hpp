@@ -307,20 +307,15 @@ __device__ inline void gemm_tile(const TileArgs& a, int tm, int tn, int b, char*
     }
     const char* Cb = (const char*)a.C + (int64_t)b * a.bsC * ((OUT == kOutF32) ? 4 : 2);
     char* Db = (char*)a.D + (int64_t)b * a.bsD * ((OUT == kOutF32) ? 4 : 2);
+    int fm[TI * TJ], fn[TI * TJ];
 #pragma unroll
-    for (int i = 0; i < TI; ++i) {
-        const int m = m0 + 128 * grp + 32 * i + 16 * h;
+    for (int i = 0; i < TI; ++i)
 #pragma unroll
         for (int j = 0; j < TJ; ++j) {
-            const int n = n0 + 64 * s + 32 * j + (lane & 31);
-            if (n < a.N && m < a.M) {
-                float v[16];
-#pragma unroll
-                for (int e = 0; e < 16; ++e) v[e] = acc[i][j][e];
-                store16<OUT>(a, Cb, Db, m, n, v, alpha, a.beta);
-            }
+            fm[i * TJ + j] = m0 + 128 * grp + 32 * i + 16 * h;
+            fn[i * TJ + j] = n0 + 64 * s + 32 * j + (lane & 31);
         }
-    }
+    store_frags<OUT, TI * TJ, (OUT == kOutF32) ? 4 : 8>(a, Cb, Db, fm, fn, &acc[0][0], alpha, a.beta);
     if (grp == 0) barrier();  // matches group 1's extra barrier
 }
 
