@@ -64,8 +64,12 @@ template <Engine E, int MODE, bool AK, bool BK, int TM, int TN, int OUT>
 inline void launch_engine(const TileArgs& a, int grid, hipStream_t s) {
     if constexpr (E == kEngV4)
         hipLaunchKernelGGL((v4::gemm_kernel<MODE, AK, BK, OUT>), dim3(grid), dim3(v4::kThreads8), 0, s, a);
-    else if constexpr (E == kEngV3)
-        hipLaunchKernelGGL((v3::gemm_kernel<MODE, AK, BK, OUT>), dim3(grid), dim3(kThreads), 0, s, a);
+    else if constexpr (E == kEngV3) {
+        if (a.flags & kFlagBeta)
+            hipLaunchKernelGGL((v3::gemm_kernel<MODE, AK, BK, OUT, true>), dim3(grid), dim3(kThreads), 0, s, a);
+        else
+            hipLaunchKernelGGL((v3::gemm_kernel<MODE, AK, BK, OUT, false>), dim3(grid), dim3(kThreads), 0, s, a);
+    }
     else if constexpr (E == kEngV2)
         hipLaunchKernelGGL((v2::gemm_kernel<MODE, AK, BK, TM, TN, OUT>), dim3(grid), dim3(kThreads), 0, s, a);
     else if constexpr (E == kEngV1)

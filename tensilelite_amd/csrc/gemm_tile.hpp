@@ -342,6 +342,38 @@ __device__ inline void store16(const TileArgs& a, const char* Cb, char* Db, int 
     }
 }
 
+// store16 for a fragment wholly inside D whose 16 C values (16-bit) were
+// loaded ahead into c0, c1; beta != 0.
+template <int OUT>
+__device__ inline void store16_pre(const TileArgs& a, char* Db, int m, int n, const float* acc, i32x4_t c0,
+                                   i32x4_t c1, float alpha, float beta) {
+    static_assert(OUT != kOutF32, "16-bit C");
+    float r[16];
+    const i32x4_t cv[2] = {c0, c1};
+#pragma unroll
+    for (int q = 0; q < 2; ++q)
+#pragma unroll
+        for (int e = 0; e < 4; ++e) {
+            const uint32_t u = (uint32_t)cv[q][e];
+            const float lo = (OUT == kOutBF16) ? bf16lo(u) : f16lo(u), hi = (OUT == kOutBF16) ? bf16hi(u) : f16hi(u);
+            r[8 * q + 2 * e] = __builtin_fmaf(alpha, acc[8 * q + 2 * e], beta * lo);
+            r[8 * q + 2 * e + 1] = __builtin_fmaf(alpha, acc[8 * q + 2 * e + 1], beta * hi);
+        }
+    char* dp = Db + ((int64_t)n * a.ldd + m) * 2;
+    const bool trunc = (a.flags & kFlagTrunc) != 0;
+#pragma unroll
+    for (int q = 0; q < 2; ++q) {
+        uint32_t w[4];
+#pragma unroll
+        for (int e = 0; e < 4; ++e) {
+            const float lo = r[8 * q + 2 * e], hi = r[8 * q + 2 * e + 1];
+            if constexpr (OUT == kOutBF16) w[e] = trunc ? pack_bf16x2_trunc(lo, hi) : pack_bf16x2_rne(lo, hi);
+            else w[e] = pack_f16x2(lo, hi);
+        }
+        *(i32x4_t*)(dp + 16 * q) = i32x4_t{(int)w[0], (int)w[1], (int)w[2], (int)w[3]};
+    }
+}
+
 // Epilogue of a wave's NF accumulator fragments held in VGPRs (the eight-wave
 // engine's 128 registers); fragment f covers rows m[f] .. m[f]+15 of column
 // n[f] (lane layout as in store16).  With beta != 0 the C slices of up to NB

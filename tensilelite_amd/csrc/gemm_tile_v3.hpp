@@ -83,7 +83,10 @@ struct RegLoader {
 template <int NM, int NR, int SYNC, int MASK>
 __device__ inline void sched_block(int nx) { v2::sched_block<NM, NR, SYNC, MASK>(nx); }
 
-template <int MODE, bool AK, bool BK, int OUT>
+// CPRE: preload C in the epilogue.  The launcher takes it only with beta != 0
+// (the beta = 0 kernel keeps the register allocation it has without it) and
+// the persistent grouped kernel not at all (its tile loop leaves no registers).
+template <int MODE, bool AK, bool BK, int OUT, bool CPRE = true>
 __device__ inline void gemm_tile(const TileArgs& a, int tm, int tn, int b, char* smem, int tid) {
     static_assert(MODE == kBF16 || MODE == kF16, "16-bit engine");
     constexpr int WT = MT / 2, T = WT / 32;  // 128 x 128 per wave, 4 x 4 MFMA tiles
@@ -196,6 +199,26 @@ __device__ inline void gemm_tile(const TileArgs& a, int tm, int tn, int b, char*
 
     const char* Cb = (const char*)a.C + (int64_t)b * a.bsC * ((OUT == kOutF32) ? 4 : 2);
     char* Db = (char*)a.D + (int64_t)b * a.bsD * ((OUT == kOutF32) ? 4 : 2);
+    // 16-bit C with beta != 0: every fragment's C slice is requested before the
+    // first is used (the loader and fragment registers are free by now); a
+    // fragment not wholly inside D reads C's first rows instead and stores
+    // element by element
+    const bool pre = CPRE && OUT != kOutF32 && (a.flags & kFlagBeta) != 0 && a.M >= 16;
+    i32x4_t cpre[T][T][2];
+    if constexpr (CPRE && OUT != kOutF32) {
+        if (pre) {
+#pragma unroll
+            for (int i = 0; i < T; ++i)
+#pragma unroll
+                for (int j = 0; j < T; ++j) {
+                    const int m = m0 + wm * WT + 32 * i + 16 * g, n = n0 + wn * WT + 32 * j + (lane & 31);
+                    const bool full = n < a.N && m + 16 <= a.M;
+                    const char* cp = full ? Cb + ((int64_t)n * a.ldc + m) * 2 : Cb;
+                    cpre[i][j][0] = *(const i32x4_t*)cp;
+                    cpre[i][j][1] = *(const i32x4_t*)(cp + 16);
+                }
+        }
+    }
 #pragma unroll
     for (int i = 0; i < T; ++i) {
         const int m = m0 + wm * WT + 32 * i + 16 * g;
@@ -206,18 +229,24 @@ __device__ inline void gemm_tile(const TileArgs& a, int tm, int tn, int b, char*
                 float v[16];
 #pragma unroll
                 for (int e = 0; e < 16; ++e) v[e] = acc[i][j][e];
+                if constexpr (CPRE && OUT != kOutF32) {
+                    if (pre && m + 16 <= a.M) {
+                        store16_pre<OUT>(a, Db, m, n, v, cpre[i][j][0], cpre[i][j][1], a.alpha, a.beta);
+                        continue;
+                    }
+                }
                 store16<OUT>(a, Cb, Db, m, n, v, a.alpha, a.beta);
             }
         }
     }
 }
 
-template <int MODE, bool AK, bool BK, int OUT>
+template <int MODE, bool AK, bool BK, int OUT, bool CPRE>
 __global__ void __launch_bounds__(kThreads, 1) gemm_kernel(TileArgs a) {
     __shared__ __attribute__((aligned(16))) char smem[LDS_BYTES];
     int tb, tm, tn;
     map_tile(blockIdx.x, gridDim.x, a, tb, tm, tn);
-    v3::gemm_tile<MODE, AK, BK, OUT>(a, tm, tn, tb, smem, threadIdx.x);
+    v3::gemm_tile<MODE, AK, BK, OUT, CPRE>(a, tm, tn, tb, smem, threadIdx.x);
 }
 
 }  // namespace v3

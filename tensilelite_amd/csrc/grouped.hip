@@ -52,7 +52,7 @@ __global__ void __launch_bounds__((GroupedEngine<MODE, AK, BK, MT_M, MT_N>::THRE
         const int r = local - grp * gsize;
         const int tm = first + r % h, tn = r / h;
         if constexpr (E::E == kEngV4) v4::gemm_tile<MODE, AK, BK, OUT>(d.a, tm, tn, 0, smem, tid);
-        else if constexpr (E::E == kEngV3) v3::gemm_tile<MODE, AK, BK, OUT>(d.a, tm, tn, 0, smem, tid);
+        else if constexpr (E::E == kEngV3) v3::gemm_tile<MODE, AK, BK, OUT, false>(d.a, tm, tn, 0, smem, tid);
         else if constexpr (E::E == kEngV2) v2::gemm_tile<MODE, AK, BK, MT_M, MT_N, OUT>(d.a, tm, tn, 0, smem, tid);
         else gemm_tile<MODE, AK, BK, MT_M, MT_N, OUT>(d.a, tm, tn, 0, smem, tid);
         // the 256 x 256 engines leave every wave past its last LDS read and
