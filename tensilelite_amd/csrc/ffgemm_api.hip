@@ -731,8 +731,7 @@ int ffgemm_grouped_plan_create(ffgemm_grouped_plan* plan, int count, const ffgem
         if (i == 0) { mode = L.mode; out = L.out; layout = L.layout; }
         else if (L.mode != mode || L.out != out || L.layout != layout)
             return fail(FFGEMM_ERR, "grouped problems must share dtype, compute and layout");
-        descs[i].a = L.args;
-        descs[i].a.group_m = 1;
+        descs[i].a = L.args;  // group height as the single GEMM's default (prepare)
     }
     // Tile schedule, longest-processing-time first: tiles in K-descending
     // order, each dealt to the workgroup with the least work so far (cost of a
@@ -747,16 +746,26 @@ int ffgemm_grouped_plan_create(ffgemm_grouped_plan* plan, int count, const ffgem
     const int grid = (int)std::max<int64_t>(1, std::min<int64_t>(total, (int64_t)cu_count() * tile_occupancy(t.m, t.n, layout)));
     std::vector<std::vector<GroupTile>> lists(grid);
     {
-        using Slot = std::pair<int64_t, int>;  // (work so far, workgroup); ties go to the lower workgroup
+        // Workgroups are dispatched to the 8 XCDs round-robin (workgroup b on
+        // XCD b % 8), each XCD with its own L2.  Ties go to the lowest rank,
+        // and ranks run XCD by XCD, so consecutive tiles, which share A and B
+        // panels, start on one XCD.
+        const int xcds = (grid % 8 == 0) ? 8 : 1;
+        auto rank = [&](int b) { return (b % xcds) * (grid / xcds) + b / xcds; };
+        using Slot = std::pair<int64_t, int>;  // (work so far, rank)
         std::priority_queue<Slot, std::vector<Slot>, std::greater<Slot>> heap;
-        for (int b = 0; b < grid; ++b) heap.push({0, b});
+        std::vector<int> of_rank(grid);
+        for (int b = 0; b < grid; ++b) {
+            of_rank[rank(b)] = b;
+            heap.push({0, rank(b)});
+        }
         for (int i : order) {
             const int nt = descs[i].a.tiles_m * descs[i].a.tiles_n;
             const int64_t cost = cdiv(probs[i].K, 64) + 2;
             for (int k = 0; k < nt; ++k) {
                 Slot w = heap.top();
                 heap.pop();
-                lists[w.second].push_back(GroupTile{i, k});
+                lists[of_rank[w.second]].push_back(GroupTile{i, k});
                 heap.push({w.first + cost, w.second});
             }
         }
