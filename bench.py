@@ -56,8 +56,9 @@ def time_launches(fn, steps, warmup, torch):
 def main():
     a = _args()
     if a.gpus != 1:
-        print(f"bench.py: this build measures one GPU; --gpus {a.gpus} is not supported "
-              "(the GEMM path is single-GPU; see DESIGN.md for how it would shard)", file=sys.stderr)
+        print(f"bench.py measures one GPU; --gpus {a.gpus} is not supported (the GEMM path runs on a "
+              "single GPU; across GPUs it would shard N, each GPU computing a block of D's columns)",
+              file=sys.stderr)
         sys.exit(2)
     import numpy as np
     import torch

@@ -96,7 +96,7 @@ float orc_f16_to_f32_quirk(uint16_t h)
 /* inc/ff_utils.h:119-122 (cvtFP32toBF16): truncation to the high 16 bits. */
 uint16_t orc_f32_to_bf16_trunc(float f) { return (uint16_t)(f2u(f) >> 16); }
 
-/* RNE fp32 -> bf16 (used by the build's device conversions). NaN stays NaN. */
+/* RNE fp32 -> bf16 (the rounding the device conversions use). NaN stays NaN. */
 uint16_t orc_f32_to_bf16_rne(float f)
 {
     uint32_t x = f2u(f);
