@@ -47,15 +47,15 @@ def test_grouped_uneven_shapes(cuda, tile):
 
 # Uneven problems with ragged K (not a multiple of 64) whose tiles outnumber the
 # workgroups of the persistent grid at every tile below (256 x 256: 265 tiles
-# on 256 workgroups; 128 x 256: 523 on 256 or 512, by layout), so workgroups run several tiles of
-# different problems one after the other.
+# on 256 workgroups; 128 x 256: 523 on 256 or 512, by layout; 64 x 64: 4066 on
+# 1024), so workgroups run several tiles of different problems one after the other.
 _LAYOUT_SHAPES = [(4096, 2048, 64), (3000, 1000, 200), (2048, 2048, 96), (72, 256, 512), (1000, 304, 64),
                   (8, 40, 24), (520, 1032, 328)]
 
 
 @pytest.mark.parametrize("layout", ["NN", "NT", "TN", "TT"])
 @pytest.mark.parametrize("tile,dtype", [((256, 256), torch.bfloat16), ((128, 256), torch.bfloat16),
-                                        ((256, 256), torch.float16)])
+                                        ((256, 256), torch.float16), ((64, 64), torch.bfloat16)])
 def test_grouped_layouts_persistent(cuda, layout, tile, dtype):
     print(f"case grouped {layout} tile={tile} {dtype}", flush=True)
     ta, tb = layout[0] == "T", layout[1] == "T"
