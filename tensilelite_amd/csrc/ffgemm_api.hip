@@ -784,7 +784,10 @@ int ffgemm_grouped_plan_create(ffgemm_grouped_plan* plan, int count, const ffgem
     P->total = total;
     P->grid = grid;
     P->mode = mode; P->out = out; P->layout = layout; P->tile = t;
-    hipGetDevice(&P->device);
+    if (hipError_t e = hipGetDevice(&P->device); e != hipSuccess) {
+        delete P;
+        return hip_fail(e, "hipGetDevice");
+    }
     if (total > 0) {
         const size_t off_first = round256(sizeof(GroupDesc) * count);
         const size_t off_sched = off_first + round256(sizeof(int32_t) * (grid + 1));
@@ -796,7 +799,7 @@ int ffgemm_grouped_plan_create(ffgemm_grouped_plan* plan, int count, const ffgem
         hipError_t e = hipMalloc(&P->d_mem, bytes);
         if (e != hipSuccess) { delete P; return hip_fail(e, "hipMalloc(grouped plan)"); }
         e = hipMemcpy(P->d_mem, host.data(), bytes, hipMemcpyHostToDevice);
-        if (e != hipSuccess) { hipFree(P->d_mem); delete P; return hip_fail(e, "hipMemcpy(grouped plan)"); }
+        if (e != hipSuccess) { (void)hipFree(P->d_mem); delete P; return hip_fail(e, "hipMemcpy(grouped plan)"); }
         P->d_descs = (const GroupDesc*)P->d_mem;
         P->d_first = (const int32_t*)((const char*)P->d_mem + off_first);
         P->d_sched = (const GroupTile*)((const char*)P->d_mem + off_sched);
@@ -833,9 +836,9 @@ int ffgemm_grouped_plan_info(ffgemm_grouped_plan plan, int64_t* total_tiles, int
 
 int ffgemm_grouped_plan_destroy(ffgemm_grouped_plan plan) {
     if (!plan) return FFGEMM_SUCCESS;
-    if (plan->d_mem) hipFree(plan->d_mem);
+    const hipError_t e = plan->d_mem ? hipFree(plan->d_mem) : hipSuccess;
     delete plan;
-    return FFGEMM_SUCCESS;
+    return e == hipSuccess ? FFGEMM_SUCCESS : hip_fail(e, "hipFree(grouped plan)");
 }
 
 }  // extern "C"

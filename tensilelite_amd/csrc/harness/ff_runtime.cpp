@@ -150,7 +150,8 @@ E_ReturnState RuntimeHip::SellectDevice(int dev) {
     info_.name = p.gcnArchName;
     info_.cu_num = p.multiProcessorCount;
     int clk_khz = 0;
-    hipDeviceGetAttribute(&clk_khz, hipDeviceAttributeClockRate, dev);
+    if (hipDeviceGetAttribute(&clk_khz, hipDeviceAttributeClockRate, dev) != hipSuccess)
+        FF_ERR("hipDeviceGetAttribute(clock rate) failed");
     info_.sclk_mhz = clk_khz / 1000.0;
     info_.global_mem = p.totalGlobalMem;
     info_.lds_per_block = p.sharedMemPerBlock;
@@ -191,28 +192,29 @@ E_ReturnState RuntimeHip::MemCopyD2H(void* h, const void* d, size_t bytes) {
 }
 
 StreamHip::StreamHip(bool en_prof) : prof_(en_prof) {
-    (void)hipStreamCreate(&s_);
-    (void)hipEventCreate(&e0_);
-    (void)hipEventCreate(&e1_);
+    ok_ = hipStreamCreate(&s_) == hipSuccess;
+    ok_ = hipEventCreate(&e0_) == hipSuccess && ok_;
+    ok_ = hipEventCreate(&e1_) == hipSuccess && ok_;
 }
 
 StreamHip::~StreamHip() {
-    hipEventDestroy(e0_);
-    hipEventDestroy(e1_);
-    hipStreamDestroy(s_);
+    if (e0_) (void)hipEventDestroy(e0_);
+    if (e1_) (void)hipEventDestroy(e1_);
+    if (s_) (void)hipStreamDestroy(s_);
 }
 
 // ff_gpu_src/ff_hip_runtime.cpp:278-320: event-timed launch.  The host waits
 // on the stop event instead of busy-polling.
 E_ReturnState StreamHip::Launch(const std::function<int(hipStream_t)>& launch) {
-    if (prof_) (void)hipEventRecord(e0_, s_);
+    if (!ok_) FF_ERR("stream or timing events could not be created");
+    if (prof_ && hipEventRecord(e0_, s_) != hipSuccess) FF_ERR("hipEventRecord failed");
     int st = launch(s_);
     if (st != 0) FF_ERR("launch failed with status %d", st);
     if (prof_) {
-        (void)hipEventRecord(e1_, s_);
+        if (hipEventRecord(e1_, s_) != hipSuccess) FF_ERR("hipEventRecord failed");
         if (hipEventSynchronize(e1_) != hipSuccess) FF_ERR("kernel execution failed");
         float ms = 0;
-        hipEventElapsedTime(&ms, e0_, e1_);
+        if (hipEventElapsedTime(&ms, e0_, e1_) != hipSuccess) FF_ERR("hipEventElapsedTime failed");
         exe_time_s_ = ms * 1e-3;
     }
     return E_ReturnState::SUCCESS;
@@ -240,19 +242,22 @@ E_ReturnState SolutionCtrlBase::RunSolution() {
     if ((r = stream_->Sync()) != E_ReturnState::SUCCESS) return r;
     E_ReturnState v = verifyResult();
     FF_LOG("Launch comb kernel for %d times...", repeat_times_);
-    hipEvent_t e0, e1;
-    hipEventCreate(&e0);
-    hipEventCreate(&e1);
-    hipEventRecord(e0, stream_->Handle());
+    struct Events {  // destroyed on every return path
+        hipEvent_t e0 = nullptr, e1 = nullptr;
+        ~Events() {
+            if (e0) (void)hipEventDestroy(e0);
+            if (e1) (void)hipEventDestroy(e1);
+        }
+    } ev;
+    if (hipEventCreate(&ev.e0) != hipSuccess || hipEventCreate(&ev.e1) != hipSuccess) FF_ERR("hipEventCreate failed");
+    if (hipEventRecord(ev.e0, stream_->Handle()) != hipSuccess) FF_ERR("hipEventRecord failed");
     for (int i = 0; i < repeat_times_; ++i) {
         if ((r = launchKernel()) != E_ReturnState::SUCCESS) return r;
     }
-    hipEventRecord(e1, stream_->Handle());
-    hipEventSynchronize(e1);
+    if (hipEventRecord(ev.e1, stream_->Handle()) != hipSuccess) FF_ERR("hipEventRecord failed");
+    if (hipEventSynchronize(ev.e1) != hipSuccess) FF_ERR("timed launches failed");
     float ms = 0;
-    hipEventElapsedTime(&ms, e0, e1);
-    hipEventDestroy(e0);
-    hipEventDestroy(e1);
+    if (hipEventElapsedTime(&ms, ev.e0, ev.e1) != hipSuccess) FF_ERR("hipEventElapsedTime failed");
     score_.ElapsedTime = ms * 1e-3 / std::max(1, repeat_times_);
     score_.Flops = score_.Calculation / score_.ElapsedTime;
     score_.Performence = score_.TheoryFlops > 0 ? score_.Flops / score_.TheoryFlops : 0;

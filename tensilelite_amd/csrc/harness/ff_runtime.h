@@ -105,6 +105,7 @@ public:
 private:
     hipStream_t s_ = nullptr;
     hipEvent_t e0_ = nullptr, e1_ = nullptr;
+    bool ok_ = false;  // stream and both events created
     bool prof_;
     double exe_time_s_ = 0;
 };
