@@ -84,8 +84,7 @@ template <int NM, int NR, int SYNC, int MASK>
 __device__ inline void sched_block(int nx) { v2::sched_block<NM, NR, SYNC, MASK>(nx); }
 
 // CPRE: preload C in the epilogue.  The launcher takes it only with beta != 0
-// (the beta = 0 kernel keeps the register allocation it has without it) and
-// the persistent grouped kernel not at all (its tile loop leaves no registers).
+// (the beta = 0 kernel keeps the register allocation it has without it).
 template <int MODE, bool AK, bool BK, int OUT, bool CPRE = true>
 __device__ inline void gemm_tile(const TileArgs& a, int tm, int tn, int b, char* smem, int tid) {
     static_assert(MODE == kBF16 || MODE == kF16, "16-bit engine");
@@ -239,6 +238,178 @@ __device__ inline void gemm_tile(const TileArgs& a, int tm, int tn, int b, char*
             }
         }
     }
+}
+
+// Persistent form for the grouped launch: runs tiles begin .. end-1 (tile_at(i,
+// a, tm, tn) names each; batch 0) as one stream of K-tiles.  The ring keeps
+// two K-tiles in flight across tile boundaries: the last two K-tiles of a tile
+// issue the loads of the next tile's first two, so the memory latency at a
+// tile start overlaps the previous tile's last K-tiles instead of following
+// its epilogue.  Each tile runs an even number of K-tiles, at least two (a
+// padding K-tile loads only zeros), so every tile starts on register slot 0
+// and A stage 0.  C is read fragment by fragment.
+template <int MODE, bool AK, bool BK, int OUT, class TileAt>
+__device__ inline void gemm_tiles_stream(TileAt&& tile_at, int begin, int end, char* smem, int tid) {
+    static_assert(MODE == kBF16 || MODE == kF16, "16-bit engine");
+    constexpr int WT = MT / 2, T = WT / 32;
+    using LA = std::conditional_t<AK, RegLoader<true, 2>, TransLoader<MT, true, 2, RB>>;
+    using LB = std::conditional_t<BK, DirectLoader<MT, false, 2, RB>, TransLoader<MT, false, 2, RB>>;
+    constexpr int NV = LA::NVMEM + LB::NVMEM;
+    constexpr int NW = LA::NSTORE + LB::NSTORE;
+    if (begin >= end) return;
+
+    const int lane = tid & 63;
+    const int wave = tid >> 6;
+    const int wm = wave & 1, wn = wave >> 1;
+    const int g = lane >> 5;
+    const int arow = wm * WT + (lane & 31);
+    const int brow = wn * WT + (lane & 31);
+    char* const sA = smem;
+    char* const sB = smem + NSA * STAGE;
+    // B's stage by position s in the stream of K-tiles
+    auto stB = [&](int s) { return sB + (s % NSB) * STAGE; };
+    auto ktiles = [](const TileArgs* a) { return max(2, ((a->K + 2 * DU - 1) / (2 * DU)) * 2); };
+
+    // the loaders address the tile `ia` (the compute tile, or the next one
+    // during the compute tile's last two K-tiles)
+    LA la;
+    LB lb;
+    int itm, itn;
+    const TileArgs* ia;
+    auto init_loaders = [&]() {
+        const TileArgs& a = *ia;
+        const int t = tid;
+        const int m0 = itm * MT, n0 = itn * MT;
+        const char* Ab = (const char*)a.A;
+        const char* Bb = (const char*)a.B;
+        if constexpr (AK) la.init(Ab + (int64_t)m0 * a.lda * 2, extent_bytes(a.M - m0, a.lda, a.K, 2), a.lda, a.M - m0, a.K, 0, t);
+        else la.init(Ab, extent_bytes(a.K, a.lda, a.M, 2), a.lda, a.M, a.K, m0, t);
+        if constexpr (BK) lb.init(Bb + (int64_t)n0 * a.ldb * 2, extent_bytes(a.N - n0, a.ldb, a.K, 2), a.ldb, a.N - n0, a.K, 0, t);
+        else lb.init(Bb, extent_bytes(a.K, a.ldb, a.N, 2), a.ldb, a.N, a.K, n0, t);
+    };
+    int ci = begin;
+    tile_at(ci, ia, itm, itn);
+    init_loaders();
+    const TileArgs* ca = ia;
+    int ctm = itm, ctn = itn, cKT = ktiles(ca);
+
+    f32x16_t acc[T][T];
+    i32x4_t fa[2][T], fb[2][T];
+    auto clear = [&]() {
+#pragma unroll
+        for (int i = 0; i < T; ++i)
+#pragma unroll
+            for (int j = 0; j < T; ++j)
+#pragma unroll
+                for (int e = 0; e < 16; ++e) acc[i][j][e] = 0.0f;
+#pragma unroll
+        for (int i = 0; i < T; ++i) {
+            fa[1][i] = i32x4_t{0, 0, 0, 0};
+            fb[1][i] = i32x4_t{0, 0, 0, 0};
+        }
+    };
+    clear();
+
+    int s = 0;
+    la.template issue<0>(0, nullptr);
+    lb.template issue<0>(0, stB(0));
+    la.template issue<1>(1, nullptr);
+    lb.template issue<1>(1, stB(1));
+    la.template store<0>(sA);
+    lb.template store<0>(stB(0));
+
+    // the K-tile at stream position s (register slot and A stage S = s & 1),
+    // as one iteration of gemm_tile's loop; the loaders fetch their tile's
+    // K-tile kl (masked when kl is past K)
+    auto step = [&](auto slot, int kl) {
+        constexpr int S = decltype(slot)::value;
+        const char* sa = sA + S * STAGE;
+        const char* sb = stB(s);
+        if constexpr (BK) wait_vmcnt<NV>();
+        lds_barrier();
+        la.template issue<S>(kl, nullptr);
+        lb.template issue<S>(kl, stB(s + 2));
+        static_for<RS>([&](auto rstep) {
+            constexpr int R = decltype(rstep)::value;
+            constexpr int CUR = R & 1, PREV = (R + 1) & 1;
+#pragma unroll
+            for (int i = 0; i < T; ++i)
+                fa[CUR][i] = *(const i32x4_t*)(sa + lds_addr<RB>(arow + 32 * i, (2 * R + g) * 16));
+#pragma unroll
+            for (int j = 0; j < T; ++j)
+                fb[CUR][j] = *(const i32x4_t*)(sb + lds_addr<RB>(brow + 32 * j, (2 * R + g) * 16));
+#pragma unroll
+            for (int i = 0; i < T; ++i)
+#pragma unroll
+                for (int j = 0; j < T; ++j) v2::mfma16<MODE>(fa[PREV][i], fb[PREV][j], acc[i][j]);
+            if constexpr (R == RS - 1) {
+                // position s+1 (the next tile's first K-tile after a tile's
+                // last) from registers into its stages
+                la.template store<S ^ 1>(sA + (S ^ 1) * STAGE);
+                lb.template store<S ^ 1>(stB(s + 1));
+                sched_block<T * T, 2 * T, R + 1, 0x200>(NW);
+            } else if constexpr (R == 0) {
+                sched_block<T * T, 2 * T, R + 1, 0x020>(NV);
+            } else {
+                sched_block<T * T, 2 * T, R + 1, 0x020>(0);
+            }
+        });
+        ++s;
+    };
+
+    while (true) {
+        const bool more = ci + 1 < end;
+        for (int k = 0; k < cKT; k += 2) {
+            // the last two K-tiles load the next tile's first two (past the
+            // last tile, K-tiles beyond K: masked)
+            int kl = k + 2;
+            if (kl == cKT && more) {
+                tile_at(ci + 1, ia, itm, itn);
+                init_loaders();
+                kl = 0;
+            }
+            step(std::integral_constant<int, 0>{}, kl);
+            step(std::integral_constant<int, 1>{}, kl + 1);
+        }
+#pragma unroll
+        for (int i = 0; i < T; ++i)
+#pragma unroll
+            for (int j = 0; j < T; ++j) v2::mfma16<MODE>(fa[1][i], fb[1][j], acc[i][j]);
+        {
+            const TileArgs& a = *ca;
+            const char* Cb = (const char*)a.C;
+            char* Db = (char*)a.D;
+#pragma unroll
+            for (int i = 0; i < T; ++i) {
+                const int m = ctm * MT + wm * WT + 32 * i + 16 * g;
+#pragma unroll
+                for (int j = 0; j < T; ++j) {
+                    const int n = ctn * MT + wn * WT + 32 * j + (lane & 31);
+                    if (n < a.N && m < a.M) {
+                        float v[16];
+#pragma unroll
+                        for (int e = 0; e < 16; ++e) v[e] = acc[i][j][e];
+                        store16<OUT>(a, Cb, Db, m, n, v, a.alpha, a.beta);
+                    }
+                }
+            }
+        }
+        if (!more) break;
+        // With stores and loads both pending, the compiler takes the memory
+        // counter as out of order and drains it to zero at the next wait it
+        // needs, which would fall inside the next tile's K loop; draining it
+        // here instead costs only the remaining latency of the next tile's
+        // second K-tile, issued one K-tile before the epilogue.
+        wait_vmcnt<0>();
+        ++ci;
+        ca = ia;
+        ctm = itm;
+        ctn = itn;
+        cKT = ktiles(ca);
+        clear();
+    }
+    // no LDS-bound load may be in flight when the workgroup ends
+    wait_vmcnt<0>();
 }
 
 template <int MODE, bool AK, bool BK, int OUT, bool CPRE>

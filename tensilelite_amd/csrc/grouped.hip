@@ -5,8 +5,10 @@
 // workgroups of a persistent grid longest-first (K descending), each tile to
 // the workgroup with the least work so far.  Workgroup b walks its own
 // contiguous slice of the schedule and runs each tile with the same tile
-// engine as the single GEMM.  Nothing is allocated or copied at launch time,
-// so the launch can be captured into a hipGraph and replayed.
+// engine as the single GEMM; the split-ring engine runs the whole slice as one
+// stream of K-tiles, so a tile's first loads overlap the previous tile's last
+// K-tiles.  Nothing is allocated or copied at launch time, so the launch can
+// be captured into a hipGraph and replayed.
 #include "grouped.hpp"
 
 namespace ffgemm {
@@ -36,6 +38,22 @@ __global__ void __launch_bounds__((GroupedEngine<MODE, AK, BK, MT_M, MT_N>::THRE
     using E = GroupedEngine<MODE, AK, BK, MT_M, MT_N>;
     __shared__ __attribute__((aligned(16))) char smem[E::LDS];
     const int end = first[blockIdx.x + 1];
+    if constexpr (E::E == kEngV3) {
+        // the split-ring engine runs the slice as one stream of K-tiles
+        auto tile_at = [&](int i, const TileArgs*& a, int& tm, int& tn) {
+            const GroupTile e = sched[i];
+            a = &descs[e.problem].a;
+            const int gm = a->group_m > 0 ? a->group_m : 1;
+            const int gsize = gm * a->tiles_n;
+            const int grp = e.tile / gsize, top = grp * gm;
+            const int h = min(a->tiles_m - top, gm);
+            const int r = e.tile - grp * gsize;
+            tm = top + r % h;
+            tn = r / h;
+        };
+        v3::gemm_tiles_stream<MODE, AK, BK, OUT>(tile_at, first[blockIdx.x], end, smem, threadIdx.x);
+        return;
+    }
     for (int i = first[blockIdx.x]; i < end; ++i) {
         // The thread index goes through an empty asm per tile, so the engine
         // derives its per-thread offsets inside the loop instead of having
@@ -52,13 +70,12 @@ __global__ void __launch_bounds__((GroupedEngine<MODE, AK, BK, MT_M, MT_N>::THRE
         const int r = local - grp * gsize;
         const int tm = first + r % h, tn = r / h;
         if constexpr (E::E == kEngV4) v4::gemm_tile<MODE, AK, BK, OUT>(d.a, tm, tn, 0, smem, tid);
-        else if constexpr (E::E == kEngV3) v3::gemm_tile<MODE, AK, BK, OUT, false>(d.a, tm, tn, 0, smem, tid);
         else if constexpr (E::E == kEngV2) v2::gemm_tile<MODE, AK, BK, MT_M, MT_N, OUT>(d.a, tm, tn, 0, smem, tid);
         else gemm_tile<MODE, AK, BK, MT_M, MT_N, OUT>(d.a, tm, tn, 0, smem, tid);
-        // the 256 x 256 engines leave every wave past its last LDS read and
+        // the eight-wave engine leaves every wave past its last LDS read and
         // with no LDS-bound load in flight; one workgroup barrier keeps the
         // next tile's prologue behind the slowest wave
-        if constexpr (E::E == kEngV4 || E::E == kEngV3) __syncthreads();
+        if constexpr (E::E == kEngV4) __syncthreads();
     }
 }
 

@@ -116,3 +116,41 @@ def test_grouped_config_graph_replays_match_eager(cuda, tile):
             assert torch.equal(got, want), f"replay {rep} differs"
             assert torch.all(d.view(p.N, p.ldd)[:, p.M:] == 7.0)
     plan.close()
+
+
+# 256 x 256 tiles with C: odd K-tile counts (K = 152 runs three K-tiles of 64,
+# padded to four in the NN streaming engine), a single K-tile (K = 40), K = 0
+# (D = beta * C), and more tiles than workgroups so tiles follow one another.
+_BETA_SHAPES = [(3000, 2048, 152), (1000, 520, 40), (520, 264, 0), (2048, 2048, 136), (296, 1032, 328),
+                (4096, 2048, 64)]
+
+
+@pytest.mark.parametrize("layout", ["NN", "TN"])
+def test_grouped_256_alpha_beta_odd_ktiles(cuda, layout):
+    print(f"case grouped {layout} 256x256 alpha/beta", flush=True)
+    ta, tb = layout[0] == "T", layout[1] == "T"
+    alpha, beta = 0.5, -1.25
+    g = torch.Generator().manual_seed(17)
+    probs, As, Bs, Cs, Ds, lds = [], [], [], [], [], []
+    for (M, N, K) in _BETA_SHAPES:
+        # K = 0: operands of 8 K-rows (never read, as every loader masks k >= K)
+        Ka = max(K, 8)
+        lda, ldb = (Ka if ta else M), (N if tb else Ka)
+        A = (torch.randn(M * Ka, generator=g) / Ka ** 0.5).to(torch.bfloat16)
+        B = (torch.randn(Ka * N, generator=g) / Ka ** 0.5).to(torch.bfloat16)
+        Cm = torch.randn(M * N, generator=g).to(torch.bfloat16)
+        As.append(A.to(cuda)); Bs.append(B.to(cuda)); Cs.append(Cm.to(cuda))
+        Ds.append(torch.full((M * N,), 7.0, dtype=torch.bfloat16, device=cuda))
+        probs.append(tl.make_problem(layout, torch.bfloat16, M, N, K, lda, ldb, M, M, alpha=alpha, beta=beta))
+        lds.append((lda, ldb))
+    plan = grouped.GroupedPlan(probs, As, Bs, Cs, Ds, tile=(256, 256))
+    info = plan.info()
+    assert info["tile"] == (256, 256) and info["total_tiles"] > info["grid"], info
+    plan.run()
+    torch.cuda.synchronize()
+    for (M, N, K), (lda, ldb), A, B, Cm, D in zip(_BETA_SHAPES, lds, As, Bs, Cs, Ds):
+        ref = oracle.gemm_f64(layout, M, N, K, alpha, A.double().cpu().numpy(), lda, B.double().cpu().numpy(), ldb,
+                              beta=beta, C=Cm.double().cpu().numpy())
+        bad, worst = elem_ok(D.float().cpu().numpy(), ref)
+        assert bad == 0, (layout, M, N, K, worst)
+    plan.close()
