@@ -90,7 +90,8 @@ typedef struct ffgemm_tile {
     int32_t depth_u;  /* K per main-loop iteration (fixed by dtype: 128 bytes of K) */
     int32_t waves_m;  /* waves along M (2) */
     int32_t waves_n;  /* waves along N (2) */
-    int32_t mfma_mn;  /* MFMA M/N width (32) */
+    int32_t mfma_mn;  /* MFMA M/N width: 32 (default), or 16 for 16x16x32 MFMAs
+                       * (bf16/fp16 at 64x64 and 128x128; -z of the reference) */
     int32_t group_m;  /* tile-raster group height (0 = default) */
     int32_t lds_buffers; /* 2 */
 } ffgemm_tile;
@@ -109,7 +110,8 @@ int ffgemm_query_tile(const ffgemm_problem *p, ffgemm_tile *out);
 
 /* Loads the tuning table that problems run without a tile consult: the
  * autotuner's best tile per (dtype, layout, M, N, K), one CSV line each,
- * `dtype,layout,M,N,K,mt_m,mt_n,group_m,waves_m,waves_n[,tflops]`.  It
+ * `dtype,layout,M,N,K,mt_m,mt_n,group_m,waves_m,waves_n,mfma_mn,tflops` (a line
+ * of 11 fields has no mfma_mn: 32 x 32 MFMAs).  It
  * replaces the table in use.  path NULL: the default, the file named by
  * FFGEMM_TUNE_DB, else tune_db.csv in the directory above the library's;
  * "" or a missing file: an empty table.  Without this call the default is

@@ -6,9 +6,11 @@ The reference sweeps padding, LDS buffers, MFMA size, MFMA tiles per wave
 ``Performance = X TFLOPs``.  Here the knobs that exist in the gfx950 kernels
 are the macro tile (MT_M x MT_N), the wave tiling (for 256 x 256: eight waves
 of 128 x 64, or four of 128 x 128 -- which also fixes the engine, its LDS
-ring depth and DepthU) and the tile-raster group height.  Results are kept in
-a CSV table keyed by (dtype, layout, M, N, K), one line
-``dtype,layout,M,N,K,mt_m,mt_n,group_m,waves_m,waves_n,tflops`` per problem.
+ring depth and DepthU), the MFMA instruction (32 x 32 x 16, or 16 x 16 x 32
+at 64 x 64 and 128 x 128) and the tile-raster group height.  Results are
+kept in a CSV table keyed by (dtype, layout, M, N, K), one line
+``dtype,layout,M,N,K,mt_m,mt_n,group_m,waves_m,waves_n,mfma_mn,tflops`` per
+problem.
 The C library reads the same table (ffgemm_tune_db_load): a problem run with
 no tile takes its entry.  ``best_tile`` looks an entry up from Python.
 """
@@ -23,10 +25,12 @@ import torch
 from . import _capi as C
 from .gemm import make_problem, make_tile, run
 
-# (mt_m, mt_n, waves_m, waves_n); waves (0, 0) is the tile's default engine
-TILES = [(256, 256, 2, 4), (256, 256, 2, 2), (256, 128, 0, 0), (128, 256, 0, 0), (128, 128, 0, 0), (64, 64, 0, 0)]
+# (mt_m, mt_n, waves_m, waves_n, mfma_mn); waves (0, 0) is the tile's default
+# engine, mfma 0 its 32 x 32 MFMAs
+TILES = [(256, 256, 2, 4, 0), (256, 256, 2, 2, 0), (256, 128, 0, 0, 0), (128, 256, 0, 0, 0), (128, 128, 0, 0, 0),
+         (128, 128, 0, 0, 16), (64, 64, 0, 0, 0), (64, 64, 0, 0, 16)]
 DB_PATH = os.environ.get("FFGEMM_TUNE_DB", os.path.join(os.path.dirname(os.path.abspath(__file__)), "tune_db.csv"))
-FIELDS = ["dtype", "layout", "M", "N", "K", "mt_m", "mt_n", "group_m", "waves_m", "waves_n", "tflops"]
+FIELDS = ["dtype", "layout", "M", "N", "K", "mt_m", "mt_n", "group_m", "waves_m", "waves_n", "mfma_mn", "tflops"]
 # an entry is recorded only when it beats the library's own choice by this
 # much: run-to-run differences on one box reach ~3 %
 MARGIN = 0.05
@@ -53,7 +57,8 @@ def load_db(path=None):
         with open(path, newline="") as f:
             for r in csv.DictReader(f):
                 db[key(int(r["dtype"]), r["layout"], int(r["M"]), int(r["N"]), int(r["K"]))] = {
-                    "tile": [int(r[k]) for k in ("mt_m", "mt_n", "group_m", "waves_m", "waves_n")],
+                    "tile": [int(r[k]) for k in ("mt_m", "mt_n", "group_m", "waves_m", "waves_n")]
+                    + [int(r.get("mfma_mn") or 0)],
                     "tflops": float(r["tflops"])}
     return db
 
@@ -69,7 +74,8 @@ def save_db(db, path=None):
             dt, lay, mnk = k.split(":")
             M, N, K = mnk.split("x")
             e = db[k]
-            w.writerow([dt, lay, M, N, K] + list(e["tile"]) + [round(e["tflops"], 1)])
+            tile = list(e["tile"]) + [0] * (6 - len(e["tile"]))
+            w.writerow([dt, lay, M, N, K] + tile + [round(e["tflops"], 1)])
     os.replace(tmp, path)
     C.check(C.lib().ffgemm_tune_db_load(path.encode(), None))
 
@@ -86,8 +92,9 @@ def tune_gemm(problem: C.Problem, A, B, Cm, D, tiles=TILES, group_ms=(0,), steps
     cands = []
     for t in tiles:
         wm, wn = (t[2], t[3]) if len(t) > 3 else (0, 0)
+        mf = t[4] if len(t) > 4 else 0
         for gm in group_ms:
-            rec = (t[0], t[1], gm, wm, wn)
+            rec = (t[0], t[1], gm, wm, wn, mf)
             try:
                 run(problem, A, B, Cm, D, make_tile(rec))
             except C.FFGemmError:
@@ -103,7 +110,8 @@ def tune_gemm(problem: C.Problem, A, B, Cm, D, tiles=TILES, group_ms=(0,), steps
     if verbose:
         for r in results:
             t = r["tile"]
-            print(f"MT{t[0]}x{t[1]} waves={t[3]}x{t[4]} gm={t[2]}: Performance = {r['tflops']:.2f} TFLOPs")
+            print(f"MT{t[0]}x{t[1]} waves={t[3]}x{t[4]} mfma={t[5] or 32} gm={t[2]}: "
+                  f"Performance = {r['tflops']:.2f} TFLOPs")
     return {"best": tuple(results[0]["tile"]) if results else None, "results": results}
 
 
@@ -117,8 +125,8 @@ def library_default(problem: C.Problem):
     finally:
         C.check(C.lib().ffgemm_tune_db_load(None, None))
     if (t.mt_m, t.mt_n) == (256, 256):
-        return (256, 256, 0, t.waves_m, t.waves_n)
-    return (t.mt_m, t.mt_n, 0, 0, 0)
+        return (256, 256, 0, t.waves_m, t.waves_n, 0)
+    return (t.mt_m, t.mt_n, 0, 0, 0, 0)
 
 
 def tune_and_record(problem, A, B, Cm, D, db=None, path=None, **kw):

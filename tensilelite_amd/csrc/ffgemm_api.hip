@@ -142,6 +142,7 @@ struct Launch {
     int layout;
     Tile tile;
     int waves;  // 0: the default engine for the tile; 4 or 8: that wave count
+    int mfma;   // 0: 32 x 32 MFMAs; 16: the 16 x 16 x 32 form
     TileArgs args;
     int64_t grid;
 };
@@ -150,8 +151,8 @@ bool aligned16(const void* p) { return ((uintptr_t)p & 15u) == 0; }
 
 // ---------------------------------------------------------------------------
 // Tuning table: the autotuner's best tile per (dtype, layout, M, N, K), kept
-// as CSV lines `dtype,layout,M,N,K,mt_m,mt_n,group_m,waves_m,waves_n[,tflops]`
-// (layout as NN/NT/TN/TT).  The successor of the best-of selection in
+// as CSV lines `dtype,layout,M,N,K,mt_m,mt_n,group_m,waves_m,waves_n,mfma_mn,tflops`
+// (layout as NN/NT/TN/TT; a line of 11 fields has no mfma_mn: 32 x 32).  The successor of the best-of selection in
 // test/perf_test.py:58-167: problems run with no tile take the table's tile.
 // ---------------------------------------------------------------------------
 using TuneKey = std::tuple<int, int, int64_t, int64_t, int64_t>;
@@ -209,6 +210,10 @@ int read_tune_table(const std::string& path, std::map<TuneKey, ffgemm_tile>& out
         memset(&t, 0, sizeof(t));
         t.mt_m = (int)x[5]; t.mt_n = (int)x[6]; t.group_m = (int)x[7];
         t.waves_m = (int)x[8]; t.waves_n = (int)x[9];
+        if (v.size() >= 12) {
+            t.mfma_mn = (int)strtol(v[10].c_str(), &end, 10);
+            if (end == v[10].c_str()) return -1;
+        }
         out[TuneKey((int)x[0], layout, x[2], x[3], x[4])] = t;
         ++n;
     }
@@ -298,6 +303,7 @@ int prepare_with(const ffgemm_problem* p, const void* A, const void* B, const vo
     Tile t = pick_tile(M, N, batch);
     int group_m = 0;
     int waves = 0;
+    int mfma = 0;
     if (tile && tile->mt_m) {
         if (!tile_supported(tile->mt_m, tile->mt_n))
             return fail(FFGEMM_ERR, "unsupported macro tile %dx%d", tile->mt_m, tile->mt_n);
@@ -312,11 +318,21 @@ int prepare_with(const ffgemm_problem* p, const void* A, const void* B, const vo
                 return fail(FFGEMM_ERR, "no %dx%d-wave kernel for MT%dx%d with this dtype and layout",
                             tile->waves_m, tile->waves_n, tile->mt_m, tile->mt_n);
         }
-        const EngineGeometry geo = engine_geometry(mode, ta, !tb, tile->mt_m, tile->mt_n, waves);
-        if ((tile->mfma_mn && tile->mfma_mn != 32) || (tile->lds_buffers && tile->lds_buffers != geo.lds_buffers) ||
+        // mfma_mn: 0 or 32 (32 x 32 MFMAs), or 16 (16 x 16 x 32, bf16 / fp16 at
+        // 64 x 64 and 128 x 128)
+        if (tile->mfma_mn == 16) {
+            if (engine_for(mode, ta, !tb, tile->mt_m, tile->mt_n, waves, 16) == kEngNone)
+                return fail(FFGEMM_ERR, "no 16x16 MFMA kernel for MT%dx%d with this dtype (bf16/fp16 at 64x64, 128x128)",
+                            tile->mt_m, tile->mt_n);
+            mfma = 16;
+        } else if (tile->mfma_mn && tile->mfma_mn != 32) {
+            return fail(FFGEMM_ERR, "unsupported mfma_mn %d (16 or 32)", tile->mfma_mn);
+        }
+        const EngineGeometry geo = engine_geometry(mode, ta, !tb, tile->mt_m, tile->mt_n, waves, mfma);
+        if ((tile->lds_buffers && tile->lds_buffers != geo.lds_buffers) ||
             (tile->depth_u && tile->depth_u != geo.depth_u))
             return fail(FFGEMM_ERR,
-                        "unsupported tile knobs for MT%dx%d (waves %dx%d, mfma 32, %d LDS stages, depth_u %d)",
+                        "unsupported tile knobs for MT%dx%d (waves %dx%d, %d LDS stages, depth_u %d)",
                         tile->mt_m, tile->mt_n, geo.waves_m, geo.waves_n, geo.lds_buffers, geo.depth_u);
         t = Tile{tile->mt_m, tile->mt_n};
         group_m = tile->group_m;
@@ -336,7 +352,7 @@ int prepare_with(const ffgemm_problem* p, const void* A, const void* B, const vo
     a.tiles_m = (int32_t)tiles_m; a.tiles_n = (int32_t)tiles_n; a.batch = (int32_t)batch;
     a.group_m = group_m > 0 ? group_m : default_group_m(tiles_m, tiles_n);
     a.flags = (use_c ? kFlagBeta : 0u) | (p->round == FFGEMM_ROUND_TRUNCATE ? kFlagTrunc : 0u);
-    L.mode = mode; L.out = out; L.layout = p->layout; L.tile = t; L.grid = grid; L.waves = waves;
+    L.mode = mode; L.out = out; L.layout = p->layout; L.tile = t; L.grid = grid; L.waves = waves; L.mfma = mfma;
     return FFGEMM_SUCCESS;
 }
 
@@ -373,11 +389,11 @@ int prepare(const ffgemm_problem* p, const void* A, const void* B, const void* C
 int launch(const Launch& L, hipStream_t s) {
     int r = -1;
     switch (L.mode) {
-        case kBF16: r = launch_bf16(L.layout, L.out, L.tile.m, L.tile.n, L.waves, L.args, (int)L.grid, s); break;
-        case kF16: r = launch_f16(L.layout, L.out, L.tile.m, L.tile.n, L.waves, L.args, (int)L.grid, s); break;
-        case kF32: r = launch_f32(L.layout, L.out, L.tile.m, L.tile.n, L.waves, L.args, (int)L.grid, s); break;
-        case kX3: r = launch_x3(L.layout, L.out, L.tile.m, L.tile.n, L.waves, L.args, (int)L.grid, s); break;
-        case kF8: r = launch_f8(L.layout, L.out, L.tile.m, L.tile.n, L.waves, L.args, (int)L.grid, s); break;
+        case kBF16: r = launch_bf16(L.layout, L.out, L.tile.m, L.tile.n, L.waves, L.mfma, L.args, (int)L.grid, s); break;
+        case kF16: r = launch_f16(L.layout, L.out, L.tile.m, L.tile.n, L.waves, L.mfma, L.args, (int)L.grid, s); break;
+        case kF32: r = launch_f32(L.layout, L.out, L.tile.m, L.tile.n, L.waves, L.mfma, L.args, (int)L.grid, s); break;
+        case kX3: r = launch_x3(L.layout, L.out, L.tile.m, L.tile.n, L.waves, L.mfma, L.args, (int)L.grid, s); break;
+        case kF8: r = launch_f8(L.layout, L.out, L.tile.m, L.tile.n, L.waves, L.mfma, L.args, (int)L.grid, s); break;
     }
     if (r != 0) return fail(FFGEMM_ERR, "no kernel for mode %d out %d layout %d tile %dx%d", L.mode, L.out,
                             L.layout, L.tile.m, L.tile.n);
@@ -470,9 +486,10 @@ int ffgemm_query_tile(const ffgemm_problem* p, ffgemm_tile* out) {
     if (out) {
         memset(out, 0, sizeof(*out));
         const EngineGeometry geo = engine_geometry(L.mode, L.layout >= FFGEMM_TN, (L.layout & 1) == 0, L.tile.m,
-                                                   L.tile.n, L.waves);
+                                                   L.tile.n, L.waves, L.mfma);
         out->mt_m = L.tile.m; out->mt_n = L.tile.n; out->depth_u = geo.depth_u;
-        out->waves_m = geo.waves_m; out->waves_n = geo.waves_n; out->mfma_mn = 32; out->group_m = L.args.group_m;
+        out->waves_m = geo.waves_m; out->waves_n = geo.waves_n; out->mfma_mn = L.mfma ? L.mfma : 32;
+        out->group_m = L.args.group_m;
         out->lds_buffers = geo.lds_buffers;
     }
     return FFGEMM_SUCCESS;
@@ -708,6 +725,8 @@ int ffgemm_grouped_plan_create(ffgemm_grouped_plan* plan, int count, const ffgem
     if (tile && tile->mt_m) {
         if (!tile_supported(tile->mt_m, tile->mt_n))
             return fail(FFGEMM_ERR, "unsupported macro tile %dx%d", tile->mt_m, tile->mt_n);
+        if (tile->mfma_mn && tile->mfma_mn != 32)
+            return fail(FFGEMM_ERR, "the grouped launch runs 32x32 MFMAs (mfma_mn %d)", tile->mfma_mn);
         t = Tile{tile->mt_m, tile->mt_n};
     } else {
         for (const Tile& c : kTiles) {

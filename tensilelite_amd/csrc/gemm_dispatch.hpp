@@ -19,10 +19,16 @@ namespace ffgemm {
 // direct-load ring shortens the latency chain of small problems); the first
 // engine (gemm_tile.hpp) for the rest and for fp32 / bf16x3.  waves = 8 or 4 asks for the eight-wave or a
 // four-wave engine (the wave-tiling knob of the tile record, which the
-// autotuner sweeps); kEngNone when the combination has no kernel.
-enum Engine : int { kEngNone = 0, kEngV1 = 1, kEngV2 = 2, kEngV3 = 3, kEngV4 = 4 };
-__host__ __device__ constexpr Engine engine_for(int mode, bool ak, bool bk, int tm, int tn, int waves = 0) {
+// autotuner sweeps); kEngNone when the combination has no kernel.  mfma = 16
+// asks for 16 x 16 x 32 MFMAs (the MFMA-instruction knob): the first engine's
+// 16 x 16 form (gemm_tile_m16) at 64 x 64 and 128 x 128 for bf16 / fp16.
+enum Engine : int { kEngNone = 0, kEngV1 = 1, kEngV2 = 2, kEngV3 = 3, kEngV4 = 4, kEngV1M16 = 5 };
+__host__ __device__ constexpr Engine engine_for(int mode, bool ak, bool bk, int tm, int tn, int waves = 0,
+                                                int mfma = 0) {
     const bool k16 = (mode == kBF16 || mode == kF16);
+    if (mfma == 16)
+        return (k16 && (waves == 0 || waves == 4) && tm == tn && (tm == 64 || tm == 128)) ? kEngV1M16 : kEngNone;
+    if (mfma != 0 && mfma != 32) return kEngNone;
     const bool big = (tm == 256 && tn == 256);
     const bool v4ok = big && (k16 || (mode == kF8 && ak && bk));
     const bool v2ok = k16 && ((tm >= 128 && tn >= 128) || (tm == 64 && tn == 64 && ak && bk));
@@ -39,9 +45,9 @@ __host__ __device__ constexpr Engine engine_for(int mode, bool ak, bool bk, int 
 struct EngineGeometry {
     int waves_m, waves_n, depth_u, lds_buffers;
 };
-inline EngineGeometry engine_geometry(int mode, bool ak, bool bk, int tm, int tn, int waves = 0) {
+inline EngineGeometry engine_geometry(int mode, bool ak, bool bk, int tm, int tn, int waves = 0, int mfma = 0) {
     const int es = mode_elem_bytes(mode);
-    switch (engine_for(mode, ak, bk, tm, tn, waves)) {
+    switch (engine_for(mode, ak, bk, tm, tn, waves, mfma)) {
         case kEngV4: return {2, 4, 128 / es, v4::NSB};
         case kEngV3: return {2, 2, v3::DU, v3::NSB};
         case kEngV2: {
@@ -55,7 +61,8 @@ inline EngineGeometry engine_geometry(int mode, bool ak, bool bk, int tm, int tn
 #undef FFGEMM_V2G
             return {2, 2, du, ns};
         }
-        case kEngV1: return {2, 2, mode_depth_u(mode), 2};
+        case kEngV1:
+        case kEngV1M16: return {2, 2, mode_depth_u(mode), 2};
         default: return {0, 0, 0, 0};
     }
 }
@@ -74,14 +81,24 @@ inline void launch_engine(const TileArgs& a, int grid, hipStream_t s) {
         hipLaunchKernelGGL((v2::gemm_kernel<MODE, AK, BK, TM, TN, OUT>), dim3(grid), dim3(kThreads), 0, s, a);
     else if constexpr (E == kEngV1)
         hipLaunchKernelGGL((gemm_kernel<MODE, AK, BK, TM, TN, OUT>), dim3(grid), dim3(kThreads), 0, s, a);
+    else if constexpr (E == kEngV1M16)
+        hipLaunchKernelGGL((gemm_kernel_m16<MODE, AK, BK, TM, TN, OUT>), dim3(grid), dim3(kThreads), 0, s, a);
 }
 
 // Launches the engine engine_for picks; only the engines reachable for this
 // (mode, layout, tile) are instantiated.
 template <int MODE, bool AK, bool BK, int OUT>
-inline int launch_layout(int mt_m, int mt_n, int waves, const TileArgs& a, int grid, hipStream_t s) {
+inline int launch_layout(int mt_m, int mt_n, int waves, int mfma, const TileArgs& a, int grid, hipStream_t s) {
 #define FFGEMM_CASE(TM_, TN_)                                                                  \
     if (mt_m == TM_ && mt_n == TN_) {                                                          \
+        constexpr Engine E16 = engine_for(MODE, AK, BK, TM_, TN_, 0, 16);                      \
+        if (mfma == 16) {                                                                      \
+            if constexpr (E16 != kEngNone) {                                                   \
+                launch_engine<E16, MODE, AK, BK, TM_, TN_, OUT>(a, grid, s);                   \
+                return 0;                                                                      \
+            }                                                                                  \
+            return -1;                                                                         \
+        }                                                                                      \
         constexpr Engine E0 = engine_for(MODE, AK, BK, TM_, TN_, 0);                           \
         constexpr Engine E4 = engine_for(MODE, AK, BK, TM_, TN_, 4);                           \
         constexpr Engine E8 = engine_for(MODE, AK, BK, TM_, TN_, 8);                           \
@@ -98,22 +115,23 @@ inline int launch_layout(int mt_m, int mt_n, int waves, const TileArgs& a, int g
 
 // layout: 0 NN, 1 NT, 2 TN, 3 TT.  A is K-contiguous for TN/TT, B for NN/TN.
 template <int MODE, int OUT>
-inline int launch_mode(int layout, int mt_m, int mt_n, int waves, const TileArgs& a, int grid, hipStream_t s) {
+inline int launch_mode(int layout, int mt_m, int mt_n, int waves, int mfma, const TileArgs& a, int grid,
+                       hipStream_t s) {
     switch (layout) {
-        case 0: return launch_layout<MODE, false, true, OUT>(mt_m, mt_n, waves, a, grid, s);
-        case 1: return launch_layout<MODE, false, false, OUT>(mt_m, mt_n, waves, a, grid, s);
-        case 2: return launch_layout<MODE, true, true, OUT>(mt_m, mt_n, waves, a, grid, s);
-        case 3: return launch_layout<MODE, true, false, OUT>(mt_m, mt_n, waves, a, grid, s);
+        case 0: return launch_layout<MODE, false, true, OUT>(mt_m, mt_n, waves, mfma, a, grid, s);
+        case 1: return launch_layout<MODE, false, false, OUT>(mt_m, mt_n, waves, mfma, a, grid, s);
+        case 2: return launch_layout<MODE, true, true, OUT>(mt_m, mt_n, waves, mfma, a, grid, s);
+        case 3: return launch_layout<MODE, true, false, OUT>(mt_m, mt_n, waves, mfma, a, grid, s);
     }
     return -1;
 }
 
 // Per-mode entry points (defined in gemm_<mode>.hip).
-// waves: 0 (default engine), 4 or 8 (see engine_for)
-int launch_bf16(int layout, int out, int mt_m, int mt_n, int waves, const TileArgs& a, int grid, hipStream_t s);
-int launch_f16(int layout, int out, int mt_m, int mt_n, int waves, const TileArgs& a, int grid, hipStream_t s);
-int launch_f32(int layout, int out, int mt_m, int mt_n, int waves, const TileArgs& a, int grid, hipStream_t s);
-int launch_x3(int layout, int out, int mt_m, int mt_n, int waves, const TileArgs& a, int grid, hipStream_t s);
-int launch_f8(int layout, int out, int mt_m, int mt_n, int waves, const TileArgs& a, int grid, hipStream_t s);
+// waves: 0 (default engine), 4 or 8; mfma: 0 / 32, or 16 (see engine_for)
+int launch_bf16(int layout, int out, int mt_m, int mt_n, int waves, int mfma, const TileArgs& a, int grid, hipStream_t s);
+int launch_f16(int layout, int out, int mt_m, int mt_n, int waves, int mfma, const TileArgs& a, int grid, hipStream_t s);
+int launch_f32(int layout, int out, int mt_m, int mt_n, int waves, int mfma, const TileArgs& a, int grid, hipStream_t s);
+int launch_x3(int layout, int out, int mt_m, int mt_n, int waves, int mfma, const TileArgs& a, int grid, hipStream_t s);
+int launch_f8(int layout, int out, int mt_m, int mt_n, int waves, int mfma, const TileArgs& a, int grid, hipStream_t s);
 
 }  // namespace ffgemm

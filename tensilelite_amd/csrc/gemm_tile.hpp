@@ -342,6 +342,61 @@ __device__ inline void store16(const TileArgs& a, const char* Cb, char* Db, int 
     }
 }
 
+// Epilogue of one lane of a 16 x 16 MFMA block: rows m .. m+3 of column n.
+template <int OUT>
+__device__ inline void store4(const TileArgs& a, const char* Cb, char* Db, int m, int n, const f32x4_t& acc,
+                              float alpha, float beta) {
+    constexpr int OS = (OUT == kOutF32) ? 4 : 2;
+    const bool use_c = (a.flags & kFlagBeta) != 0;
+    const bool trunc = (a.flags & kFlagTrunc) != 0;
+    char* dp = Db + ((int64_t)n * a.ldd + m) * OS;
+    const char* cp = Cb + ((int64_t)n * a.ldc + m) * OS;
+    if (m + 4 <= a.M) {
+        float c[4] = {0.0f, 0.0f, 0.0f, 0.0f};
+        if (use_c) {
+            if constexpr (OUT == kOutF32) {
+                const f32x4_t cv = *(const f32x4_t*)cp;
+                c[0] = cv[0]; c[1] = cv[1]; c[2] = cv[2]; c[3] = cv[3];
+            } else {
+                const i32x2_t cv = *(const i32x2_t*)cp;
+                const uint32_t u0 = (uint32_t)cv[0], u1 = (uint32_t)cv[1];
+                c[0] = (OUT == kOutBF16) ? bf16lo(u0) : f16lo(u0);
+                c[1] = (OUT == kOutBF16) ? bf16hi(u0) : f16hi(u0);
+                c[2] = (OUT == kOutBF16) ? bf16lo(u1) : f16lo(u1);
+                c[3] = (OUT == kOutBF16) ? bf16hi(u1) : f16hi(u1);
+            }
+        }
+        float r[4];
+#pragma unroll
+        for (int e = 0; e < 4; ++e) r[e] = use_c ? __builtin_fmaf(alpha, acc[e], beta * c[e]) : alpha * acc[e];
+        if constexpr (OUT == kOutF32) {
+            *(f32x4_t*)dp = f32x4_t{r[0], r[1], r[2], r[3]};
+        } else if constexpr (OUT == kOutBF16) {
+            const uint32_t w0 = trunc ? pack_bf16x2_trunc(r[0], r[1]) : pack_bf16x2_rne(r[0], r[1]);
+            const uint32_t w1 = trunc ? pack_bf16x2_trunc(r[2], r[3]) : pack_bf16x2_rne(r[2], r[3]);
+            *(i32x2_t*)dp = i32x2_t{(int)w0, (int)w1};
+        } else {
+            *(i32x2_t*)dp = i32x2_t{(int)pack_f16x2(r[0], r[1]), (int)pack_f16x2(r[2], r[3])};
+        }
+    } else {
+        for (int e = 0; e < 4 && m + e < a.M; ++e) {
+            float c = 0.0f;
+            if (use_c) {
+                if constexpr (OUT == kOutF32) c = ((const float*)cp)[e];
+                else {
+                    const uint16_t u = ((const uint16_t*)cp)[e];
+                    c = (OUT == kOutBF16) ? bf16lo(u) : f16lo(u);
+                }
+            }
+            const float r = use_c ? __builtin_fmaf(alpha, acc[e], beta * c) : alpha * acc[e];
+            if constexpr (OUT == kOutF32) ((float*)dp)[e] = r;
+            else if constexpr (OUT == kOutBF16)
+                ((uint16_t*)dp)[e] = (uint16_t)((trunc ? pack_bf16x2_trunc(r, 0.f) : pack_bf16x2_rne(r, 0.f)) & 0xffffu);
+            else ((uint16_t*)dp)[e] = (uint16_t)(pack_f16x2(r, 0.f) & 0xffffu);
+        }
+    }
+}
+
 // store16 for a fragment wholly inside D whose 16 C values (16-bit) were
 // loaded ahead into c0, c1; beta != 0.
 template <int OUT>
@@ -637,6 +692,126 @@ __global__ void __launch_bounds__(kThreads, 1) gemm_kernel(TileArgs a) {
     int tb, tm, tn;
     map_tile(blockIdx.x, gridDim.x, a, tb, tm, tn);
     gemm_tile<MODE, AK, BK, MT_M, MT_N, OUT>(a, tm, tn, tb, smem, threadIdx.x);
+}
+
+// ---------------------------------------------------------------------------
+// The same tile with 16 x 16 x 32 MFMAs (tile record mfma_mn = 16; bf16 and
+// fp16).  Lane l supplies row (column) l % 16 of its operand block and the 8
+// consecutive K of 16-byte chunk l / 16 of the read step, so one MFMA covers
+// 32 K and a 128-byte K-tile takes two read steps; the result block gives lane
+// l rows 4 * (l / 16) .. +3 of column l % 16.  Those rows are consecutive in
+// the natural row order, so A is staged unpermuted.  The pipeline is the
+// 32 x 32 tile's with two read steps.
+// ---------------------------------------------------------------------------
+template <int MODE>
+__device__ inline void mfma16x16(const i32x4_t& a, const i32x4_t& b, f32x4_t& acc) {
+    if constexpr (MODE == kBF16)
+        acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(__builtin_bit_cast(bf16x8_t, a), __builtin_bit_cast(bf16x8_t, b),
+                                                      acc, 0, 0, 0);
+    else
+        acc = __builtin_amdgcn_mfma_f32_16x16x32_f16(__builtin_bit_cast(f16x8_t, a), __builtin_bit_cast(f16x8_t, b),
+                                                     acc, 0, 0, 0);
+}
+
+template <int MODE, bool AK, bool BK, int MT_M, int MT_N, int OUT>
+__device__ inline void gemm_tile_m16(const TileArgs& a, int tm, int tn, int b, char* smem, int tid) {
+    static_assert(MODE == kBF16 || MODE == kF16, "16 x 16 x 32 MFMA: 16-bit operands");
+    constexpr int ES = 2;
+    constexpr int DU = mode_depth_u(MODE);
+    constexpr int WT_M = MT_M / 2, WT_N = MT_N / 2;
+    constexpr int TM = WT_M / 16, TN = WT_N / 16;
+    constexpr int STAGE = (MT_M + MT_N) * kRowBytes;
+    static_assert(kRowBytes == 128 && DU == 64, "two read steps of 32 K per K-tile");
+
+    const int lane = tid & 63;
+    const int wave = tid >> 6;
+    const int wm = wave & 1, wn = wave >> 1;
+    const int m0 = tm * MT_M, n0 = tn * MT_N;
+
+    const char* Ab = (const char*)a.A + (int64_t)b * a.bsA * ES;
+    const char* Bb = (const char*)a.B + (int64_t)b * a.bsB * ES;
+    Loader<MODE, AK, MT_M, false> la;
+    Loader<MODE, BK, MT_N, false> lb;
+    if constexpr (AK) la.init(Ab + (int64_t)m0 * a.lda * ES, extent_bytes(a.M - m0, a.lda, a.K, ES), a.lda, a.M - m0, a.K, 0, tid);
+    else la.init(Ab, extent_bytes(a.K, a.lda, a.M, ES), a.lda, a.M, a.K, m0, tid);
+    if constexpr (BK) lb.init(Bb + (int64_t)n0 * a.ldb * ES, extent_bytes(a.N - n0, a.ldb, a.K, ES), a.ldb, a.N - n0, a.K, 0, tid);
+    else lb.init(Bb, extent_bytes(a.K, a.ldb, a.N, ES), a.ldb, a.N, a.K, n0, tid);
+
+    f32x4_t acc[TM][TN];
+#pragma unroll
+    for (int i = 0; i < TM; ++i)
+#pragma unroll
+        for (int j = 0; j < TN; ++j) acc[i][j] = f32x4_t{0.0f, 0.0f, 0.0f, 0.0f};
+
+    const int KT = (a.K + DU - 1) / DU;
+    const int q = lane >> 4;
+    const int arow = wm * WT_M + (lane & 15);
+    const int brow = wn * WT_N + (lane & 15);
+    // fragment of read step rs (K 32*rs .. 32*rs+31 of the K-tile)
+    auto rd = [&](const char* st, int row, int rs) { return *(const i32x4_t*)(st + lds_addr(row, (4 * rs + q) * 16)); };
+
+    i32x4_t fa[2][TM], fb[2][TN];
+    la.load(0);
+    lb.load(0);
+    la.store(smem);
+    lb.store(smem + MT_M * kRowBytes);
+    lds_barrier();
+#pragma unroll
+    for (int i = 0; i < TM; ++i) fa[0][i] = rd(smem, arow + 16 * i, 0);
+#pragma unroll
+    for (int j = 0; j < TN; ++j) fb[0][j] = rd(smem + MT_M * kRowBytes, brow + 16 * j, 0);
+
+    for (int kt = 0; kt < KT; ++kt) {
+        const char* sa = smem + (kt & 1) * STAGE;
+        const char* sb = sa + MT_M * kRowBytes;
+        char* na = smem + ((kt + 1) & 1) * STAGE;
+        la.load(kt + 1);
+        lb.load(kt + 1);
+        // read step 0: fragments of step 1 load while the MFMAs of step 0 run,
+        // then K-tile kt+1 goes from registers to the other stage
+#pragma unroll
+        for (int i = 0; i < TM; ++i) fa[1][i] = rd(sa, arow + 16 * i, 1);
+#pragma unroll
+        for (int j = 0; j < TN; ++j) fb[1][j] = rd(sb, brow + 16 * j, 1);
+#pragma unroll
+        for (int i = 0; i < TM; ++i)
+#pragma unroll
+            for (int j = 0; j < TN; ++j) mfma16x16<MODE>(fa[0][i], fb[0][j], acc[i][j]);
+        la.store(na);
+        lb.store(na + MT_M * kRowBytes);
+        sched_step<MODE, TM, TN>(la.NLD + lb.NLD, la.NSTORE + lb.NSTORE);
+        lds_barrier();
+        // read step 1's MFMAs run while the first fragments of K-tile kt+1 load
+#pragma unroll
+        for (int i = 0; i < TM; ++i) fa[0][i] = rd(na, arow + 16 * i, 0);
+#pragma unroll
+        for (int j = 0; j < TN; ++j) fb[0][j] = rd(na + MT_M * kRowBytes, brow + 16 * j, 0);
+#pragma unroll
+        for (int i = 0; i < TM; ++i)
+#pragma unroll
+            for (int j = 0; j < TN; ++j) mfma16x16<MODE>(fa[1][i], fb[1][j], acc[i][j]);
+        sched_step<MODE, TM, TN>(0, 0);
+    }
+
+    const char* Cb = (const char*)a.C + (int64_t)b * a.bsC * ((OUT == kOutF32) ? 4 : 2);
+    char* Db = (char*)a.D + (int64_t)b * a.bsD * ((OUT == kOutF32) ? 4 : 2);
+#pragma unroll
+    for (int i = 0; i < TM; ++i) {
+        const int m = m0 + wm * WT_M + 16 * i + 4 * q;
+#pragma unroll
+        for (int j = 0; j < TN; ++j) {
+            const int n = n0 + wn * WT_N + 16 * j + (lane & 15);
+            if (n < a.N && m < a.M) store4<OUT>(a, Cb, Db, m, n, acc[i][j], a.alpha, a.beta);
+        }
+    }
+}
+
+template <int MODE, bool AK, bool BK, int MT_M, int MT_N, int OUT>
+__global__ void __launch_bounds__(kThreads, 1) gemm_kernel_m16(TileArgs a) {
+    __shared__ __attribute__((aligned(16))) char smem[2 * (MT_M + MT_N) * kRowBytes];
+    int tb, tm, tn;
+    map_tile(blockIdx.x, gridDim.x, a, tb, tm, tn);
+    gemm_tile_m16<MODE, AK, BK, MT_M, MT_N, OUT>(a, tm, tn, tb, smem, threadIdx.x);
 }
 
 }  // namespace ffgemm

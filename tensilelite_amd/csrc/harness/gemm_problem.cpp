@@ -143,6 +143,8 @@ protected:
             int z = (int)ca->Int("mfma-mn");
             tile_.mt_m = z * (int)ca->Int("mfma-pttn0") * (int)ca->Int("wave-pttn0");
             tile_.mt_n = z * (int)ca->Int("mfma-pttn1") * (int)ca->Int("wave-pttn1");
+            // -z 16 asks for 16 x 16 x 32 MFMAs (the tile record's MFMA knob)
+            if (ca->Given("mfma-mn")) tile_.mfma_mn = z;
             // -x -y are the waves per group: the wave tiling of the tile record
             if (ca->Given("wave-pttn0") || ca->Given("wave-pttn1")) {
                 tile_.waves_m = (int)ca->Int("wave-pttn0");

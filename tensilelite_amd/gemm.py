@@ -60,10 +60,13 @@ def make_problem(layout, dtype, M, N, K, lda, ldb, ldc, ldd, alpha=1.0, beta=0.0
 
 
 def make_tile(tile) -> C.Tile | None:
-    """Tile record from (mt_m, mt_n[, group_m[, waves_m, waves_n]]) or a Tile.
+    """Tile record from (mt_m, mt_n[, group_m[, waves_m, waves_n[, mfma_mn]]])
+    or a Tile.
 
     waves_m x waves_n picks the wave tiling: 2 x 4 the eight-wave engine,
-    2 x 2 a four-wave one, 0 x 0 (or absent) the tile's default."""
+    2 x 2 a four-wave one, 0 x 0 (or absent) the tile's default.  mfma_mn 16
+    picks 16 x 16 x 32 MFMAs (bf16 / fp16 at 64 x 64 and 128 x 128); 0 or 32
+    the 32 x 32 ones."""
     if tile is None:
         return None
     if isinstance(tile, C.Tile):
@@ -74,6 +77,8 @@ def make_tile(tile) -> C.Tile | None:
         t.group_m = int(tile[2])
     if len(tile) > 4:
         t.waves_m, t.waves_n = int(tile[3]), int(tile[4])
+    if len(tile) > 5:
+        t.mfma_mn = int(tile[5])
     return t
 
 

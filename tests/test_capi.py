@@ -228,6 +228,35 @@ def test_wave_tiling_knob_rejects(layout, dtype, tile, waves):
         tl.query_tile(p, t)
 
 
+@pytest.mark.parametrize("layout", ["NN", "NT", "TN", "TT"])
+@pytest.mark.parametrize("dtype", [C.BF16, C.F16])
+@pytest.mark.parametrize("mt", [64, 128])
+def test_mfma_knob_16x16(layout, dtype, mt):
+    """mfma_mn = 16 selects the 16 x 16 x 32 MFMA form of the first engine
+    (four waves of 2 x 2, K = 64 per stage, two LDS stages) and is reported back."""
+    t = C.Tile()
+    t.mt_m, t.mt_n, t.mfma_mn = mt, mt, 16
+    r = tl.query_tile(_p(layout, dtype, 1000, 520, 328), t)
+    assert (r.mt_m, r.mt_n, r.mfma_mn, r.waves_m, r.waves_n, r.depth_u, r.lds_buffers) == (mt, mt, 16, 2, 2, 64, 2)
+    t32 = C.Tile()
+    t32.mt_m, t32.mt_n, t32.mfma_mn = mt, mt, 32
+    assert tl.query_tile(_p(layout, dtype, 1000, 520, 328), t32).mfma_mn == 32
+
+
+@pytest.mark.parametrize("dtype,tile,mfma,waves", [
+    (C.BF16, (256, 256), 16, (0, 0)),  # 16 x 16 only at 64 x 64 and 128 x 128
+    (C.BF16, (128, 256), 16, (0, 0)),
+    (C.F32, (64, 64), 16, (0, 0)),     # 16-bit operands only
+    (C.BF16, (64, 64), 16, (2, 4)),    # four waves only
+    (C.BF16, (64, 64), 8, (0, 0)),     # no such instruction width
+])
+def test_mfma_knob_rejects(dtype, tile, mfma, waves):
+    t = C.Tile()
+    t.mt_m, t.mt_n, t.mfma_mn, t.waves_m, t.waves_n = tile[0], tile[1], mfma, waves[0], waves[1]
+    with pytest.raises(C.FFGemmError, match="RTN_ERR"):
+        tl.query_tile(_p("NN", dtype, 1000, 520, 328), t)
+
+
 def test_tuned_tile_lookup(tmp_path, monkeypatch):
     """tile='tuned' reads the autotuner table (a missing entry means the library's choice)."""
     import importlib
@@ -235,11 +264,15 @@ def test_tuned_tile_lookup(tmp_path, monkeypatch):
     G = importlib.import_module("tensilelite_amd.gemm")   # the module (the package re-exports gemm())
     monkeypatch.setattr(autotune, "DB_PATH", str(tmp_path / "db.csv"))
     try:
-        autotune.save_db({autotune.key(C.BF16, "TN", 4096, 4096, 4096): {"tile": [256, 256, 0, 2, 2], "tflops": 1.0}})
+        autotune.save_db({autotune.key(C.BF16, "TN", 4096, 4096, 4096): {"tile": [256, 256, 0, 2, 2], "tflops": 1.0},
+                          autotune.key(C.F16, "NN", 480, 512, 512): {"tile": [64, 64, 0, 0, 0, 16], "tflops": 1.0}})
         p = _p("TN", C.BF16, 4096, 4096, 4096)
-        assert list(G._tuned(p)) == [256, 256, 0, 2, 2]
+        assert list(G._tuned(p)) == [256, 256, 0, 2, 2, 0]
         t = tl.query_tile(p, G.make_tile(G._tuned(p)))
-        assert (t.waves_m, t.waves_n) == (2, 2)
+        assert (t.waves_m, t.waves_n, t.mfma_mn) == (2, 2, 32)
+        p16 = _p("NN", C.F16, 480, 512, 512)
+        assert list(G._tuned(p16)) == [64, 64, 0, 0, 0, 16]
+        assert tl.query_tile(p16).mfma_mn == 16        # the C library reads the same table
         assert G._tuned(_p("TN", C.BF16, 1024, 1024, 1024)) is None
     finally:
         C.check(C.lib().ffgemm_tune_db_load(b"", None))
@@ -258,15 +291,18 @@ def test_tune_table_consulted_by_the_library(tmp_path):
     f.write_text("dtype,layout,M,N,K,mt_m,mt_n,group_m,waves_m,waves_n,tflops\n"
                  f"{C.BF16},TN,4096,4096,4096,256,256,0,2,2,1000.0\n"
                  f"{C.F32},NN,4096,4096,4096,256,256,0,2,4,100.0\n"      # no eight-wave fp32 kernel
-                 f"{C.BF16},NN,480,512,512,128,128,4,0,0,50.0\n")
+                 f"{C.BF16},NN,480,512,512,128,128,4,0,0,50.0\n"
+                 f"{C.F16},TT,480,512,512,64,64,0,0,0,16,40.0\n")      # with the mfma_mn field
     try:
-        assert _load_table(f) == (C.SUCCESS, 3)
+        assert _load_table(f) == (C.SUCCESS, 4)
         t = tl.query_tile(_p("TN", C.BF16, 4096, 4096, 4096))          # the table's four-wave engine
         assert (t.mt_m, t.mt_n, t.waves_m, t.waves_n) == (256, 256, 2, 2)
         t = tl.query_tile(_p("NN", C.F32, 4096, 4096, 4096))           # passed over: the default
         assert (t.mt_m, t.mt_n) == (256, 256) and t.waves_n == 2
         t = tl.query_tile(_p("NN", C.BF16, 480, 512, 512))
-        assert (t.mt_m, t.mt_n, t.group_m) == (128, 128, 4)
+        assert (t.mt_m, t.mt_n, t.group_m, t.mfma_mn) == (128, 128, 4, 32)
+        t = tl.query_tile(_p("TT", C.F16, 480, 512, 512))
+        assert (t.mt_m, t.mt_n, t.mfma_mn) == (64, 64, 16)
         t = tl.query_tile(_p("TN", C.BF16, 4096, 4096, 1024))          # not in the table
         assert (t.mt_m, t.mt_n, t.waves_n) == (256, 256, 4)
         bad = tmp_path / "bad.csv"
@@ -293,8 +329,8 @@ def test_committed_tune_table_is_valid():
             M, N, K = (int(x) for x in mnk.split("x"))
             p = _p(lay, int(dt), M, N, K)
             t = tl.query_tile(p)
-            mt_m, mt_n, gm, wm, wn = e["tile"]
-            assert (t.mt_m, t.mt_n) == (mt_m, mt_n), k
+            mt_m, mt_n, gm, wm, wn, mf = e["tile"]
+            assert (t.mt_m, t.mt_n, t.mfma_mn) == (mt_m, mt_n, mf or 32), k
             if wm:
                 assert (t.waves_m, t.waves_n) == (wm, wn), k
             if gm:

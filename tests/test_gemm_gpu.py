@@ -660,3 +660,63 @@ def test_256_wave_tilings(cuda, name, layout, waves):
     ref = _f64_ref(layout, M, N, K, bits_to_f32(Ab, name), ra, bits_to_f32(Bb, name), rb)
     bad, worst = elem_ok(D.float().cpu().numpy(), ref)
     assert bad == 0, worst
+
+
+@pytest.mark.parametrize("name", ["bf16", "f16"])
+@pytest.mark.parametrize("layout", LAYOUTS)
+@pytest.mark.parametrize("mt", [64, 128])
+def test_16x16_mfma_form(cuda, name, layout, mt):
+    """The tile record's MFMA knob (mfma_mn = 16: 16 x 16 x 32 MFMAs) against
+    the FP64 oracle: a partial K-tile and edges in M and N, then ragged M, N
+    (rows past M inside a lane's four) with padded leading dimensions and
+    alpha / beta with C; D's padding rows stay untouched."""
+    tile = (mt, mt, 0, 0, 0, 16)
+    _say(f"case 16x16 MFMA {name} {layout} tile={mt}x{mt}")
+    rng = np.random.default_rng(21)
+    M, N, K = 328, 272, 200
+    (ra, ca), (rb, cb) = op_dims(layout, M, N, K)
+    Ab = quantize_bits(colmajor(rng, ra, ca, kind="normal", scale=1 / np.sqrt(K)), name)
+    Bb = quantize_bits(colmajor(rng, rb, cb, kind="normal", scale=1 / np.sqrt(K)), name)
+    D = tl.gemm(layout, M, N, K, _dev16(Ab, name, cuda), _dev16(Bb, name, cuda), tile=tile)
+    torch.cuda.synchronize()
+    ref = _f64_ref(layout, M, N, K, bits_to_f32(Ab, name), ra, bits_to_f32(Bb, name), rb)
+    bad, worst = elem_ok(D.float().cpu().numpy(), ref)
+    assert bad == 0, f"{bad} elements off (worst scaled error {worst:.3g})"
+
+    M, N, K = 65, 37, 136
+    (ra, ca), (rb, cb) = op_dims(layout, M, N, K)
+    lda, ldb, ldc, ldd = (ra + 7) // 8 * 8, (rb + 7) // 8 * 8, M + 15, (M + 7) // 8 * 8 + 8
+    ldc = (ldc + 7) // 8 * 8
+    A = colmajor(rng, ra, ca, lda, kind="normal")
+    B = colmajor(rng, rb, cb, ldb, kind="normal")
+    Cm = colmajor(rng, M, N, ldc)
+    bits = [quantize_bits(x, name) for x in (A, B, Cm, np.full(ldd * N, 7.0, dtype=np.float32))]
+    Ad, Bd, Cd, Dd = (_dev16(b, name, cuda) for b in bits)
+    alpha, beta = 0.5, -1.5
+    tl.gemm(layout, M, N, K, Ad, Bd, Cd, Dd, lda=lda, ldb=ldb, ldc=ldc, ldd=ldd, alpha=alpha, beta=beta, tile=tile)
+    torch.cuda.synchronize()
+    got = Dd.float().cpu().numpy()
+    ref = _f64_ref(layout, M, N, K, bits_to_f32(bits[0], name), lda, bits_to_f32(bits[1], name), ldb, alpha, beta,
+                   bits_to_f32(bits[2], name), ldc, ldd)
+    bad, worst = elem_ok(extract(got, M, N, ldd), extract(ref, M, N, ldd))
+    assert bad == 0, worst
+    assert np.all(got.reshape(N, ldd)[:, M:] == 7.0)
+
+
+@pytest.mark.parametrize("layout", LAYOUTS)
+@pytest.mark.parametrize("mt", [64, 128])
+def test_16x16_mfma_form_f32_out(cuda, layout, mt):
+    """bf16 inputs, fp32 D and C through the 16 x 16 MFMA form."""
+    M, N, K = 144, 104, 264
+    _say(f"case 16x16 MFMA bf16->f32 {layout} tile={mt}x{mt}")
+    rng = np.random.default_rng(22)
+    (ra, ca), (rb, cb) = op_dims(layout, M, N, K)
+    Ab = quantize_bits(colmajor(rng, ra, ca, kind="normal", scale=1 / np.sqrt(K)), "bf16")
+    Bb = quantize_bits(colmajor(rng, rb, cb, kind="normal", scale=1 / np.sqrt(K)), "bf16")
+    Cm = colmajor(rng, M, N)
+    Cd = torch.from_numpy(Cm).to(cuda)
+    D = tl.gemm(layout, M, N, K, _dev16(Ab, "bf16", cuda), _dev16(Bb, "bf16", cuda), Cd, alpha=2.0, beta=0.25,
+                dtype_out=torch.float32, tile=(mt, mt, 0, 0, 0, 16))
+    torch.cuda.synchronize()
+    ref = _f64_ref(layout, M, N, K, bits_to_f32(Ab, "bf16"), ra, bits_to_f32(Bb, "bf16"), rb, 2.0, 0.25, Cm)
+    assert rel_fro(D.cpu().numpy(), ref) <= 1e-5

@@ -154,3 +154,14 @@ def test_grouped_256_alpha_beta_odd_ktiles(cuda, layout):
         bad, worst = elem_ok(D.float().cpu().numpy(), ref)
         assert bad == 0, (layout, M, N, K, worst)
     plan.close()
+
+
+def test_grouped_rejects_16x16_mfma(cuda):
+    """The grouped launch runs 32 x 32 MFMAs only; a record asking for 16 x 16 is refused."""
+    from tensilelite_amd import _capi as C
+    A = torch.zeros(64 * 64, dtype=torch.bfloat16, device=cuda)
+    B = torch.zeros(64 * 64, dtype=torch.bfloat16, device=cuda)
+    D = torch.zeros(64 * 64, dtype=torch.bfloat16, device=cuda)
+    p = tl.make_problem("NN", torch.bfloat16, 64, 64, 64, 64, 64, 64, 64)
+    with pytest.raises(C.FFGemmError, match="RTN_ERR"):
+        grouped.GroupedPlan([p], [A], [B], None, [D], tile=(64, 64, 0, 0, 0, 16))
