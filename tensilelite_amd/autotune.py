@@ -225,8 +225,10 @@ def main(argv=None):
         best = r["results"][0]
         ms_def = next((x["ms"] for x in r["results"] if tuple(x["tile"]) == r["default"]), float("nan"))
         rec = "recorded" if key(dt, lay, M, N, K) in db else "library default kept"
+        m16 = next((x for x in r["results"] if x["tile"][5] == 16), None)
+        m16s = f", best 16x16 MFMA {tuple(m16['tile'])} {m16['ms'] * 1e3:.1f} us" if m16 else ""
         print(f"dtype {dt} {lay} {M}x{N}x{K}: best {tuple(best['tile'])} {best['ms'] * 1e3:.1f} us, "
-              f"default {r['default']} {ms_def * 1e3:.1f} us: {rec}", flush=True)
+              f"default {r['default']} {ms_def * 1e3:.1f} us{m16s}: {rec}", flush=True)
         del A, B, D
 
 
