@@ -49,9 +49,13 @@ struct Cfg {
     static constexpr int PF = NS - 1;                  // K-tiles in flight
 };
 
+// 16-byte chunk swizzle of a stage row.  For 64-byte rows the fragment reads
+// (32 consecutive rows at one chunk per half-wave) are conflict-free with
+// bits 2-4 of the row (SQ_LDS_BANK_CONFLICT 0 on gfx950); the earlier
+// ((r >> 1) ^ (r >> 3)) & 3 cost 4 conflict cycles per ds_read_b128.
 template <int RB>
 __host__ __device__ constexpr int swz(int r) {
-    return RB == 128 ? ((r ^ (r >> 3)) & 7) : (((r >> 1) ^ (r >> 3)) & 3);
+    return RB == 128 ? ((r ^ (r >> 3)) & 7) : (((r >> 2) ^ (r >> 3)) & 3);
 }
 template <int RB>
 __host__ __device__ constexpr int lds_addr(int row, int off) {
