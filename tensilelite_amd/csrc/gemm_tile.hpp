@@ -3,7 +3,8 @@
 // Replaces the fetch/math wave pair of src/gemmMfmaKernelWriter.h
 // (fetch_loop :1226-1316, math_loop :1403-1447, fetch_c :1822-1906,
 // mfma_mfma_pang_with_store :1572-1768) with a single-role design: every wave
-// both loads (16-byte buffer loads into VGPRs, one K-tile ahead) and computes
+// both loads (16-byte buffer loads into VGPRs, one K-tile ahead, two for
+// 64 x 64 tiles) and computes
 // (MFMA from a two-stage LDS ring), which on gfx950 keeps the MFMA pipe busy
 // without spending half the waves on address generation.
 #pragma once
@@ -18,7 +19,8 @@ namespace ffgemm {
 // Otherwise it is M/N-contiguous and each thread loads KB consecutive K-rows
 // of an 8 (16-bit) or 4 (32-bit) element column strip and transposes it in
 // registers.
-template <int MODE, bool KC, int MT, bool PERM>
+// PF: register slots, i.e. K-tiles whose loads can be in flight at once.
+template <int MODE, bool KC, int MT, bool PERM, int PF = 1>
 struct Loader {
     static constexpr int ES = mode_elem_bytes(MODE);
     static constexpr int DU = mode_depth_u(MODE);
@@ -41,7 +43,7 @@ struct Loader {
     uint32_t kstep;         // bytes to advance per K-tile
     int32_t K;
     int tid;                // thread index in the workgroup
-    i32x4_t v[NLD];
+    i32x4_t v[PF][NLD];
 
     // ld: leading dimension (elements); rows: M or N; row0: first row of tile
     __device__ inline void init(const void* base, uint32_t extent_bytes, int32_t ld, int32_t rows,
@@ -73,18 +75,21 @@ struct Loader {
         }
     }
 
-    // Issue the 16-byte loads of K-tile kt.
+    // Issue the 16-byte loads of K-tile kt into register slot S.
+    template <int S = 0>
     __device__ inline void load(int kt) {
         const int k0 = kt * DU;
 #pragma unroll
         for (int i = 0; i < NLD; ++i) {
             bool ok = (kidx[i] + k0) < K && voff[i] != kOOB;
             uint32_t off = ok ? voff[i] + (uint32_t)kt * kstep : kOOB;
-            v[i] = buf_load16(rsrc, off);
+            v[S][i] = buf_load16(rsrc, off);
         }
     }
 
-    // Write the loaded K-tile into one LDS stage (K-contiguous 128-byte rows).
+    // Write the K-tile in register slot S into one LDS stage (K-contiguous
+    // 128-byte rows).
+    template <int S = 0>
     __device__ inline void store(char* stage) const {
         const int t = tid;
         if constexpr (KC) {
@@ -94,11 +99,11 @@ struct Loader {
                 int r = (t >> 3) + 32 * i;
                 int lr = PERM ? perm_a_row(r) : r;
                 if constexpr (!X3) {
-                    *(i32x4_t*)(stage + lds_addr(lr, c * 16)) = v[i];
+                    *(i32x4_t*)(stage + lds_addr(lr, c * 16)) = v[S][i];
                 } else {
                     // 4 fp32 of K [4c, 4c+4) -> hi bf16 at bytes 8c, lo at 64 + 8c
                     i32x2_t hi, lo;
-                    split4(v[i], hi, lo);
+                    split4(v[S][i], hi, lo);
                     *(i32x2_t*)(stage + lds_addr(lr, c * 8)) = hi;
                     *(i32x2_t*)(stage + lds_addr(lr, 64 + c * 8)) = lo;
                 }
@@ -115,8 +120,8 @@ struct Loader {
                     uint32_t w[KB / 2];
 #pragma unroll
                     for (int p = 0; p < KB / 2; ++p) {
-                        uint32_t a = (uint32_t)v[2 * p][i >> 1];
-                        uint32_t b = (uint32_t)v[2 * p + 1][i >> 1];
+                        uint32_t a = (uint32_t)v[S][2 * p][i >> 1];
+                        uint32_t b = (uint32_t)v[S][2 * p + 1][i >> 1];
                         w[p] = (i & 1) ? ((a >> 16) | (b & 0xffff0000u)) : ((a & 0xffffu) | (b << 16));
                     }
                     char* dst = stage + lds_addr(lr, kg * KB * 2);
@@ -133,18 +138,18 @@ struct Loader {
                     if constexpr (!X3) {
 #pragma unroll
                         for (int h = 0; h < KB / 4; ++h) {
-                            i32x4_t o = {v[4 * h][i], v[4 * h + 1][i], v[4 * h + 2][i], v[4 * h + 3][i]};
+                            i32x4_t o = {v[S][4 * h][i], v[S][4 * h + 1][i], v[S][4 * h + 2][i], v[S][4 * h + 3][i]};
                             *(i32x4_t*)(stage + lds_addr(lr, kg * KB * 4 + 16 * h)) = o;
                         }
                         if constexpr (KB == 2) {
-                            *(i32x2_t*)(stage + lds_addr(lr, kg * 8)) = i32x2_t{v[0][i], v[1][i]};
+                            *(i32x2_t*)(stage + lds_addr(lr, kg * 8)) = i32x2_t{v[S][0][i], v[S][1][i]};
                         }
                     } else {
                         uint32_t hw[KB / 2], lw[KB / 2];
 #pragma unroll
                         for (int p = 0; p < KB / 2; ++p) {
-                            float a = as_float(v[2 * p][i]);
-                            float b = as_float(v[2 * p + 1][i]);
+                            float a = as_float(v[S][2 * p][i]);
+                            float b = as_float(v[S][2 * p + 1][i]);
                             split2(a, b, hw[p], lw[p]);
                         }
                         char* dh = stage + lds_addr(lr, kg * KB * 2);
@@ -564,8 +569,12 @@ __device__ inline void gemm_tile(const TileArgs& a, int tm, int tn, int b, char*
     const char* Ab = (const char*)a.A + (int64_t)b * a.bsA * ES;
     const char* Bb = (const char*)a.B + (int64_t)b * a.bsB * ES;
 
-    Loader<MODE, AK, MT_M, true> la;
-    Loader<MODE, BK, MT_N, false> lb;
+    // 64 x 64 tiles keep two K-tiles of loads in flight (a register slot
+    // each): a small problem is a chain of K-tiles per tile, each waiting on
+    // the memory latency when only the next K-tile's loads are in flight
+    constexpr int PF = (MT_M * MT_N <= 64 * 64) ? 2 : 1;
+    Loader<MODE, AK, MT_M, true, PF> la;
+    Loader<MODE, BK, MT_N, false, PF> lb;
     // A: K-contiguous = rows M of length lda; M-contiguous = rows K of length lda
     // A K-contiguous operand's buffer starts at the tile's first row, so its
     // offsets stay within the tile's rows however large the matrix is.
@@ -595,23 +604,31 @@ __device__ inline void gemm_tile(const TileArgs& a, int tm, int tn, int b, char*
     //                 fragments of K-tile kt+1 are read, so the MFMA pipe does
     //                 not drain across the barrier.
     // Loads past K are masked to zero, so the loop has no branches.
+    // With PF = 2, K-tile kt sits in register slot kt & 1 until it is
+    // written to LDS; iteration kt refills that slot with K-tile kt + 2.
     Frag<MODE> fa[2][TM], fb[2][TN];
-    la.load(0);
-    lb.load(0);
-    la.store(smem);
-    lb.store(smem + MT_M * kRowBytes);
+    la.template load<0>(0);
+    lb.template load<0>(0);
+    if constexpr (PF == 2) {
+        la.template load<1>(1);
+        lb.template load<1>(1);
+    }
+    la.template store<0>(smem);
+    lb.template store<0>(smem + MT_M * kRowBytes);
     lds_barrier();
 #pragma unroll
     for (int i = 0; i < TM; ++i) read_frag<MODE>(smem, arow + 32 * i, g, 0, fa[0][i]);
 #pragma unroll
     for (int j = 0; j < TN; ++j) read_frag<MODE>(smem + MT_M * kRowBytes, brow + 32 * j, g, 0, fb[0][j]);
 
-    for (int kt = 0; kt < KT; ++kt) {
+    auto iteration = [&](auto slot, int kt) {
+        constexpr int S = decltype(slot)::value;     // slot of K-tile kt (0 when PF = 1)
+        constexpr int SN = (PF == 2) ? (S ^ 1) : 0;  // slot of K-tile kt + 1
         const char* sa = smem + (kt & 1) * STAGE;
         const char* sb = sa + MT_M * kRowBytes;
         char* na = smem + ((kt + 1) & 1) * STAGE;
-        la.load(kt + 1);
-        lb.load(kt + 1);
+        la.template load<S>(kt + PF);
+        lb.template load<S>(kt + PF);
 #pragma unroll
         for (int rs = 0; rs + 1 < RS; ++rs) {
 #pragma unroll
@@ -623,8 +640,8 @@ __device__ inline void gemm_tile(const TileArgs& a, int tm, int tn, int b, char*
 #pragma unroll
                 for (int j = 0; j < TN; ++j) mfma<MODE>(fa[rs & 1][i], fb[rs & 1][j], acc[i][j]);
             if (rs + 2 == RS) {
-                la.store(na);
-                lb.store(na + MT_M * kRowBytes);
+                la.template store<SN>(na);
+                lb.template store<SN>(na + MT_M * kRowBytes);
             }
             sched_step<MODE, TM, TN>(rs == 0 ? la.NLD + lb.NLD : 0,
                                      rs + 2 == RS ? la.NSTORE + lb.NSTORE : 0);
@@ -639,6 +656,16 @@ __device__ inline void gemm_tile(const TileArgs& a, int tm, int tn, int b, char*
 #pragma unroll
             for (int j = 0; j < TN; ++j) mfma<MODE>(fa[(RS - 1) & 1][i], fb[(RS - 1) & 1][j], acc[i][j]);
         sched_step<MODE, TM, TN>(0, 0);
+    };
+    if constexpr (PF == 1) {
+        for (int kt = 0; kt < KT; ++kt) iteration(std::integral_constant<int, 0>{}, kt);
+    } else {
+        int kt = 0;
+        for (; kt + 2 <= KT; kt += 2) {
+            iteration(std::integral_constant<int, 0>{}, kt);
+            iteration(std::integral_constant<int, 1>{}, kt + 1);
+        }
+        if (kt < KT) iteration(std::integral_constant<int, 0>{}, kt);
     }
 
     // epilogue
@@ -701,7 +728,7 @@ __global__ void __launch_bounds__(kThreads, 1) gemm_kernel(TileArgs a) {
 // 32 K and a 128-byte K-tile takes two read steps; the result block gives lane
 // l rows 4 * (l / 16) .. +3 of column l % 16.  Those rows are consecutive in
 // the natural row order, so A is staged unpermuted.  The pipeline is the
-// 32 x 32 tile's with two read steps.
+// 32 x 32 tile's with two read steps and two K-tiles of loads in flight.
 // ---------------------------------------------------------------------------
 template <int MODE>
 __device__ inline void mfma16x16(const i32x4_t& a, const i32x4_t& b, f32x4_t& acc) {
@@ -730,8 +757,9 @@ __device__ inline void gemm_tile_m16(const TileArgs& a, int tm, int tn, int b, c
 
     const char* Ab = (const char*)a.A + (int64_t)b * a.bsA * ES;
     const char* Bb = (const char*)a.B + (int64_t)b * a.bsB * ES;
-    Loader<MODE, AK, MT_M, false> la;
-    Loader<MODE, BK, MT_N, false> lb;
+    // two K-tiles of loads in flight (register slot kt & 1 holds K-tile kt)
+    Loader<MODE, AK, MT_M, false, 2> la;
+    Loader<MODE, BK, MT_N, false, 2> lb;
     if constexpr (AK) la.init(Ab + (int64_t)m0 * a.lda * ES, extent_bytes(a.M - m0, a.lda, a.K, ES), a.lda, a.M - m0, a.K, 0, tid);
     else la.init(Ab, extent_bytes(a.K, a.lda, a.M, ES), a.lda, a.M, a.K, m0, tid);
     if constexpr (BK) lb.init(Bb + (int64_t)n0 * a.ldb * ES, extent_bytes(a.N - n0, a.ldb, a.K, ES), a.ldb, a.N - n0, a.K, 0, tid);
@@ -751,22 +779,25 @@ __device__ inline void gemm_tile_m16(const TileArgs& a, int tm, int tn, int b, c
     auto rd = [&](const char* st, int row, int rs) { return *(const i32x4_t*)(st + lds_addr(row, (4 * rs + q) * 16)); };
 
     i32x4_t fa[2][TM], fb[2][TN];
-    la.load(0);
-    lb.load(0);
-    la.store(smem);
-    lb.store(smem + MT_M * kRowBytes);
+    la.template load<0>(0);
+    lb.template load<0>(0);
+    la.template load<1>(1);
+    lb.template load<1>(1);
+    la.template store<0>(smem);
+    lb.template store<0>(smem + MT_M * kRowBytes);
     lds_barrier();
 #pragma unroll
     for (int i = 0; i < TM; ++i) fa[0][i] = rd(smem, arow + 16 * i, 0);
 #pragma unroll
     for (int j = 0; j < TN; ++j) fb[0][j] = rd(smem + MT_M * kRowBytes, brow + 16 * j, 0);
 
-    for (int kt = 0; kt < KT; ++kt) {
+    auto iteration = [&](auto slot, int kt) {
+        constexpr int S = decltype(slot)::value;
         const char* sa = smem + (kt & 1) * STAGE;
         const char* sb = sa + MT_M * kRowBytes;
         char* na = smem + ((kt + 1) & 1) * STAGE;
-        la.load(kt + 1);
-        lb.load(kt + 1);
+        la.template load<S>(kt + 2);
+        lb.template load<S>(kt + 2);
         // read step 0: fragments of step 1 load while the MFMAs of step 0 run,
         // then K-tile kt+1 goes from registers to the other stage
 #pragma unroll
@@ -777,8 +808,8 @@ __device__ inline void gemm_tile_m16(const TileArgs& a, int tm, int tn, int b, c
         for (int i = 0; i < TM; ++i)
 #pragma unroll
             for (int j = 0; j < TN; ++j) mfma16x16<MODE>(fa[0][i], fb[0][j], acc[i][j]);
-        la.store(na);
-        lb.store(na + MT_M * kRowBytes);
+        la.template store<S ^ 1>(na);
+        lb.template store<S ^ 1>(na + MT_M * kRowBytes);
         sched_step<MODE, TM, TN>(la.NLD + lb.NLD, la.NSTORE + lb.NSTORE);
         lds_barrier();
         // read step 1's MFMAs run while the first fragments of K-tile kt+1 load
@@ -791,7 +822,13 @@ __device__ inline void gemm_tile_m16(const TileArgs& a, int tm, int tn, int b, c
 #pragma unroll
             for (int j = 0; j < TN; ++j) mfma16x16<MODE>(fa[1][i], fb[1][j], acc[i][j]);
         sched_step<MODE, TM, TN>(0, 0);
+    };
+    int kt = 0;
+    for (; kt + 2 <= KT; kt += 2) {
+        iteration(std::integral_constant<int, 0>{}, kt);
+        iteration(std::integral_constant<int, 1>{}, kt + 1);
     }
+    if (kt < KT) iteration(std::integral_constant<int, 0>{}, kt);
 
     const char* Cb = (const char*)a.C + (int64_t)b * a.bsC * ((OUT == kOutF32) ? 4 : 2);
     char* Db = (char*)a.D + (int64_t)b * a.bsD * ((OUT == kOutF32) ? 4 : 2);
