@@ -4,7 +4,7 @@
 // (fetch_loop :1226-1316, math_loop :1403-1447, fetch_c :1822-1906,
 // mfma_mfma_pang_with_store :1572-1768) with a single-role design: every wave
 // both loads (16-byte buffer loads into VGPRs, one K-tile ahead, two for
-// 64 x 64 tiles) and computes
+// tiles up to 128 x 128) and computes
 // (MFMA from a two-stage LDS ring), which on gfx950 keeps the MFMA pipe busy
 // without spending half the waves on address generation.
 #pragma once
@@ -569,10 +569,10 @@ __device__ inline void gemm_tile(const TileArgs& a, int tm, int tn, int b, char*
     const char* Ab = (const char*)a.A + (int64_t)b * a.bsA * ES;
     const char* Bb = (const char*)a.B + (int64_t)b * a.bsB * ES;
 
-    // 64 x 64 tiles keep two K-tiles of loads in flight (a register slot
-    // each): a small problem is a chain of K-tiles per tile, each waiting on
-    // the memory latency when only the next K-tile's loads are in flight
-    constexpr int PF = (MT_M * MT_N <= 64 * 64) ? 2 : 1;
+    // Tiles up to 128 x 128 keep two K-tiles of loads in flight (a register
+    // slot each): each K-tile otherwise waits on the memory latency with only
+    // the next K-tile's loads in flight.  The larger tiles lack the registers.
+    constexpr int PF = (MT_M * MT_N <= 128 * 128) ? 2 : 1;
     Loader<MODE, AK, MT_M, true, PF> la;
     Loader<MODE, BK, MT_N, false, PF> lb;
     // A: K-contiguous = rows M of length lda; M-contiguous = rows K of length lda
