@@ -15,7 +15,7 @@ namespace ffgemm {
 // ping-pong engine (v4::) for 256 x 256 TN and NT (M/N-contiguous operands
 // through transposing LDS reads) and fp8 TN; the split-ring engine (v3::) for
 // 256 x 256 NN; the pipelined 4-wave engine (v2::) for 256 x 256 TT, the
-// other 16-bit tiles of 128 and up, and 16-bit 64 x 64 TN (its three-stage
+// other 16-bit tiles of 128 and up, and 16-bit 64 x 64 TN (its four-stage
 // direct-load ring shortens the latency chain of small problems); the first
 // engine (gemm_tile.hpp) for the rest and for fp32 / bf16x3.  waves = 8 or 4 asks for the eight-wave or a
 // four-wave engine (the wave-tiling knob of the tile record, which the
