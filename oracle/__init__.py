@@ -1,8 +1,8 @@
 """CPU oracle for the gemmMfma path -- TEST INFRASTRUCTURE ONLY.
 
 Only ``tests/``, ``__graft_entry__.smoke()`` and ``bench.py``'s
-``cpu_baseline`` leg import this package, and only as the checker (or the
-timed CPU baseline).  The product package ``tensilelite_amd`` never imports
+``cpu_baseline`` leg import this package, and only to check results (or as
+the timed CPU baseline).  The product package ``tensilelite_amd`` never imports
 it.  The C restatement lives in ``oracle.c`` (each function cites the
 TensileLite file:line it follows); this module loads it with ctypes and adds
 numpy conveniences.

@@ -2,7 +2,7 @@
  * CPU oracle for the gemmMfma path -- TEST INFRASTRUCTURE ONLY.
  *
  * Only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg may
- * load this library, and only as the checker (or the timed CPU baseline).
+ * load this library, and only to check results (or as the timed CPU baseline).
  * The product (tensilelite_amd, libffgemm.so) never links or calls it.
  *
  * Every function restates the reference's algorithm for this path and cites
