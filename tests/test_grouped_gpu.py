@@ -8,7 +8,7 @@ import torch
 import oracle
 import tensilelite_amd as tl
 from tensilelite_amd import grouped
-from tests._util import elem_ok
+from tests._util import elem_ok, extract
 
 pytestmark = pytest.mark.gpu
 
@@ -153,6 +153,43 @@ def test_grouped_256_alpha_beta_odd_ktiles(cuda, layout):
                               beta=beta, C=Cm.double().cpu().numpy())
         bad, worst = elem_ok(D.float().cpu().numpy(), ref)
         assert bad == 0, (layout, M, N, K, worst)
+    plan.close()
+
+
+# NN at 256 x 256 (the streaming split-ring engine) with ragged last rows of
+# tiles: M mod 256 at 1..8, 128 and 136, M <= 128, ragged N, K = 0, alpha and
+# beta with C, two runs of one plan.
+_HALF_SHAPES = [(8, 300, 64), (128, 512, 200), (136, 256, 96), (384, 776, 512), (392, 256, 72), (640, 1000, 0),
+                (264, 64, 1024), (2148, 1536, 256), (1, 40, 16), (3000, 2048, 152)]
+
+
+@pytest.mark.parametrize("dtype", [torch.bfloat16, torch.float16])
+def test_grouped_256_ragged_rows(cuda, dtype):
+    print(f"case grouped NN 256x256 ragged rows {dtype}", flush=True)
+    alpha, beta = 1.5, 0.5
+    g = torch.Generator().manual_seed(23)
+    probs, As, Bs, Cs, Ds, lds = [], [], [], [], [], []
+    for (M, N, K) in _HALF_SHAPES:
+        Ka, ld = max(K, 8), (M + 7) // 8 * 8
+        A = (torch.randn(ld * Ka, generator=g) / Ka ** 0.5).to(dtype)
+        B = (torch.randn(Ka * N, generator=g) / Ka ** 0.5).to(dtype)
+        Cm = torch.randn(ld * N, generator=g).to(dtype)
+        As.append(A.to(cuda)); Bs.append(B.to(cuda)); Cs.append(Cm.to(cuda))
+        Ds.append(torch.full((ld * N,), 7.0, dtype=dtype, device=cuda))
+        probs.append(tl.make_problem("NN", dtype, M, N, K, ld, Ka, ld, ld, alpha=alpha, beta=beta))
+        lds.append((ld, Ka))
+    plan = grouped.GroupedPlan(probs, As, Bs, Cs, Ds, tile=(256, 256))
+    assert plan.info()["total_tiles"] == sum(-(-M // 256) * -(-N // 256) for (M, N, K) in _HALF_SHAPES)
+    for rep in range(2):  # a second run over the same schedule gives the same result
+        plan.run()
+        torch.cuda.synchronize()
+        for (M, N, K), (ld, Ka), A, B, Cm, D in zip(_HALF_SHAPES, lds, As, Bs, Cs, Ds):
+            ref = oracle.gemm_f64("NN", M, N, K, alpha, A.double().cpu().numpy(), ld, B.double().cpu().numpy(), Ka,
+                                  beta, Cm.double().cpu().numpy(), ld, ld)
+            got = D.float().cpu().numpy()
+            bad, worst = elem_ok(extract(got, M, N, ld), extract(ref, M, N, ld))
+            assert bad == 0, (rep, M, N, K, worst)
+            assert np.all(got.reshape(N, ld)[:, M:] == 7.0), (M, N, K, "padding rows written")
     plan.close()
 
 
