@@ -301,6 +301,9 @@ int prepare_with(const ffgemm_problem* p, const void* A, const void* B, const vo
         return fail(FFGEMM_ERR, "operand exceeds 2 GiB per batch (K-contiguous operands: 257 rows of ld)");
 
     Tile t = pick_tile(M, N, batch);
+    // the in-register bf16x3 split does not fit 256 x 256 without spilling
+    // (8192^3 on MI355X: 165-203 TFLOP/s there, 284-289 at 128 x 256)
+    if (mode == kX3 && t.m == 256 && t.n == 256) t = Tile{128, 256};
     int group_m = 0;
     int waves = 0;
     int mfma = 0;
