@@ -24,6 +24,7 @@ import time
 
 PEAK_TFLOPS = {"bf16": 2500.0, "f16": 2500.0, "f8": 5000.0, "f32": 157.3, "f64": 78.6}
 HBM_GBPS = 8000.0
+MAX_SCLK_MHZ = 2400.0   # the clock the published peaks assume
 
 
 def _args():
@@ -34,23 +35,107 @@ def _args():
     ap.add_argument("--no-extras", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--only", default="", help="run only this extra (profiling)")
+    ap.add_argument("--preload-s", type=float, default=1.5,
+                    help="untimed seconds of the headline kernel before its warm-up (clock settling)")
     return ap.parse_args()
 
 
-def time_launches(fn, steps, warmup, torch):
-    """Average duration of fn() over `steps` back-to-back launches, HIP events
-    on the stream the kernels are launched on (torch's current stream)."""
+def time_launches(fn, steps, warmup, torch, min_ms=0.0):
+    """Average duration of fn() over back-to-back launches, HIP events on the
+    stream the kernels are launched on (torch's current stream).  `steps`
+    launches, or, when min_ms is set, as many as fill at least min_ms (a block
+    of `steps` first measures one launch): a short block falls inside the
+    power manager's clock ramp and reads low."""
     for _ in range(warmup):
         fn()
     torch.cuda.synchronize()
     s = torch.cuda.current_stream()
-    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
-    e0.record(s)
-    for _ in range(steps):
-        fn()
-    e1.record(s)
-    torch.cuda.synchronize()
-    return e0.elapsed_time(e1) / steps  # ms
+
+    def block(n):
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record(s)
+        for _ in range(n):
+            fn()
+        e1.record(s)
+        torch.cuda.synchronize()
+        return e0.elapsed_time(e1)
+
+    ms = block(steps)
+    if min_ms and ms < min_ms:
+        n = max(steps, int(math.ceil(steps * min_ms / max(ms, 1e-3))))
+        return block(n) / n
+    return ms / steps  # ms
+
+
+def preload(fn, seconds, torch):
+    """Untimed launches of fn() for `seconds` of wall time: the clock settles
+    at the power limit's steady state for this load before the timed region
+    (the driver's few warm-up launches fall inside the ramp).  Returns the
+    launch count."""
+    n = 0
+    t0 = time.perf_counter()
+    while time.perf_counter() - t0 < seconds:
+        for _ in range(50):
+            fn()
+        n += 50
+        torch.cuda.synchronize()
+    return n
+
+
+class ClockSampler:
+    """Samples the GPU's graphics clock (amdsmi GPU metrics) from a background
+    thread; mean() is the average of the samples inside a window.  Every
+    method is a no-op when amdsmi is unavailable."""
+
+    def __init__(self, period_s=0.002):
+        import threading
+        self.samples = []
+        self.ok = False
+        self._stop = threading.Event()
+        try:
+            import amdsmi
+            amdsmi.amdsmi_init()
+            self._smi = amdsmi
+            self._h = amdsmi.amdsmi_get_processor_handles()[0]
+            self._read()
+            self.ok = True
+        except Exception:
+            return
+        self._period = period_s
+        self._t = threading.Thread(target=self._run, daemon=True)
+        self._t.start()
+
+    def _read(self):
+        m = self._smi.amdsmi_get_gpu_metrics_info(self._h)
+        v = m.get("current_gfxclks")
+        if isinstance(v, (list, tuple)):
+            v = [x for x in v if isinstance(x, (int, float)) and 0 < x < 10000]
+            return sum(v) / len(v) if v else None
+        v = m.get("current_gfxclk", m.get("average_gfxclk_frequency"))
+        return float(v) if isinstance(v, (int, float)) and 0 < v < 10000 else None
+
+    def _run(self):
+        while not self._stop.is_set():
+            try:
+                v = self._read()
+                if v:
+                    self.samples.append((time.perf_counter(), v))
+            except Exception:
+                pass
+            time.sleep(self._period)
+
+    def mean(self, t0, t1):
+        v = [c for t, c in self.samples if t0 <= t <= t1]
+        return round(sum(v) / len(v), 1) if v else None
+
+    def stop(self):
+        if self.ok:
+            self._stop.set()
+            self._t.join(timeout=1)
+            try:
+                self._smi.amdsmi_shut_down()
+            except Exception:
+                pass
 
 
 def main():
@@ -86,7 +171,13 @@ def main():
     tile = tl.query_tile(prob)
     fn = lambda: tl.run(prob, A, B, None, D)
 
-    # timed region: barrier + sync on both sides (one rank)
+    # clock settling: untimed launches until the power manager's clock is at
+    # its steady state for this load, then the W warm-up launches and the K
+    # timed ones (barrier + sync on both sides; one rank)
+    clk = ClockSampler()
+    tp = time.perf_counter()
+    n_pre = preload(fn, a.preload_s, torch)
+    tp1 = time.perf_counter()
     for _ in range(a.warmup):
         fn()
     torch.cuda.synchronize()
@@ -98,9 +189,14 @@ def main():
         fn()
     e1.record(s)
     torch.cuda.synchronize()
-    wall = time.perf_counter() - t0
+    t1 = time.perf_counter()
+    wall = t1 - t0
     ms = e0.elapsed_time(e1) / a.steps
     tflops = flop / (ms * 1e-3) / 1e12
+    time.sleep(0.01)
+    # the clock over the last half second of the preload and the timed region
+    sclk = clk.mean(max(tp, tp1 - 0.5), time.perf_counter())
+    clk.stop()
 
     out = {
         "metric": "TFLOP/s & % MFMA peak, fp16/bf16/fp8 4096^3; bf16x3 vs fp32 at equal error",
@@ -123,7 +219,14 @@ def main():
                      "unit": "TFLOP/s", "frac": round(tflops / PEAK_TFLOPS["bf16"], 4),
                      "traffic": profiled_traffic(), "traffic_source": TRAFFIC_PROFILE,
                      "kernel": "ffgemm::v4::gemm_kernel<bf16,TN,%dx%d>" % (tile.mt_m, tile.mt_n),
-                     "algorithmic_flop_per_launch": flop, "wall_s": round(wall, 4)},
+                     "algorithmic_flop_per_launch": flop, "wall_s": round(wall, 4),
+                     "sclk_mhz": sclk,
+                     "peak_at_measured_clock": round(PEAK_TFLOPS["bf16"] * sclk / MAX_SCLK_MHZ, 1) if sclk else None,
+                     "frac_at_measured_clock": round(tflops / (PEAK_TFLOPS["bf16"] * sclk / MAX_SCLK_MHZ), 4)
+                     if sclk else None},
+        "preload": {"seconds": a.preload_s, "launches": n_pre,
+                    "note": "untimed launches of the headline kernel before the warm-up, so the timed "
+                            "launches run at the steady-state clock of this load"},
     }
 
     extras = {}
@@ -186,7 +289,7 @@ def run_extras(a, torch, np, tl, C, dev):
     for _ in range(3):
         for name, dt, p in forms:
             A, B, C0, D = bufs[dt]
-            times[name].append(time_launches(lambda: tl.run(p, A, B, C0, D), max(10, steps // 3), 3, torch))
+            times[name].append(time_launches(lambda: tl.run(p, A, B, C0, D), max(10, steps // 3), 3, torch, min_ms=60))
     for name, _, _ in forms:
         ms = sorted(times[name])[1]
         tf = flop / (ms * 1e-3) / 1e12
@@ -197,7 +300,7 @@ def run_extras(a, torch, np, tl, C, dev):
     if want("torch_bf16_4096"):
         x = torch.randn(M, K, dtype=torch.bfloat16, device=dev)
         y = torch.randn(K, N, dtype=torch.bfloat16, device=dev)
-        ms = time_launches(lambda: torch.mm(x, y), steps, 5, torch)
+        ms = time_launches(lambda: torch.mm(x, y), steps, 5, torch, min_ms=200)
         ex["torch_bf16_4096_hipblaslt"] = {"tflops": round(flop / (ms * 1e-3) / 1e12, 1), "ms": round(ms, 4)}
         del x, y
 
@@ -212,7 +315,7 @@ def run_extras(a, torch, np, tl, C, dev):
         qb, sb = tl.fp8.quantize(y, rows=S, k=S, ldx=S, trans_x=True)
         D = torch.empty(S * S, dtype=torch.float32, device=dev)
         p = tl.make_problem("TN", C.F8E4M3, S, S, S, S, S, S, S, scaleA=sa, scaleB=sb, dtype_out=C.F32)
-        ms = time_launches(lambda: tl.run(p, qa, qb, None, D), max(5, steps // 4), 3, torch)
+        ms = time_launches(lambda: tl.run(p, qa, qb, None, D), max(5, steps // 4), 3, torch, min_ms=300)
         tf = 2.0 * S ** 3 / (ms * 1e-3) / 1e12
         # error of the whole form (quantisation included) against the FP64 GPU
         # product of the unquantised fp32 inputs, on 256 output columns: D[m, n]
@@ -241,7 +344,7 @@ def run_extras(a, torch, np, tl, C, dev):
             nws = tl.workspace_size(p)
             ws = torch.empty(nws, dtype=torch.uint8, device=dev) if nws else None
             # a few untimed launches first: each form runs at its own clock under the power limit
-            ms = time_launches(lambda: tl.run(p, A, B, None, D, workspace=ws), 5, 3, torch)
+            ms = time_launches(lambda: tl.run(p, A, B, None, D, workspace=ws), 5, 3, torch, min_ms=300)
             del ws
             tf = 2.0 * S ** 3 / (ms * 1e-3) / 1e12
             # error vs the FP64 GPU oracle product on 256 sampled columns
@@ -272,8 +375,8 @@ def run_extras(a, torch, np, tl, C, dev):
             b_.grad = None
             c_.backward(dc)
 
-        ms_fb = time_launches(step, max(5, steps // 4), 3, torch)
-        ms_f = time_launches(lambda: matmul(a_.detach(), b_.detach()), max(5, steps // 4), 3, torch)
+        ms_fb = time_launches(step, max(5, steps // 4), 3, torch, min_ms=300)
+        ms_f = time_launches(lambda: matmul(a_.detach(), b_.detach()), max(5, steps // 4), 3, torch, min_ms=300)
         fl = 2.0 * Mg * Ng * Kg
         ex["autograd_bf16_8192x4096x4096"] = {"fwd_tflops": round(fl / (ms_f * 1e-3) / 1e12, 1),
                                               "fwd_bwd_tflops": round(3 * fl / (ms_fb * 1e-3) / 1e12, 1),
@@ -316,16 +419,22 @@ def run_extras(a, torch, np, tl, C, dev):
     # grouped: 64 problems, one persistent launch, hipGraph replay
     if want("grouped_64"):
         from tensilelite_amd import grouped
-        ex["grouped_64"] = grouped.bench_config(dev, steps=max(10, steps // 4))
+        # blocks of at least 100 launches (about 40 ms each), so no block sits in the clock ramp
+        ex["grouped_64"] = grouped.bench_config(dev, steps=max(100, steps // 4))
     return ex
 
 
 def cpu_baseline(np, gpu_ms):
     """The oracle's restatement of the reference CPU GEMM (src/gemmMfma.cpp:
-    325-346), fp32, k innermost, timed on this host: a bounded sample of the
-    headline workload (1024 of the 4096 output columns of the 4096^3 TN GEMM,
-    about 10 s of CPU work) on all cores, plus the reference's own CPU case
-    (480x512x512) on 1 core."""
+    325-346: fp32, k innermost, compiled -O2 without fast-math), timed on this
+    host's cores:
+      * value: a bounded sample of the headline workload on all cores, 1024 of
+        the 4096 output columns of the 4096^3 TN GEMM (about 15 core-seconds);
+      * the full 4096^3 GEMM on one core, extrapolated from 128 of its output
+        columns (each column is an independent M x K x 1 product with the same
+        loop, so time scales with the column count);
+      * the reference's own CPU case, fp32 NN 480x512x512 (U[-1, 1], seed 0),
+        on one core (median of 3) and on all cores."""
     sys.path.insert(0, os.path.dirname(os.path.abspath(__file__)))
     import oracle
 
@@ -339,23 +448,39 @@ def cpu_baseline(np, gpu_ms):
     oracle.gemm_f32("TN", M, ncols, K, 1.0, A, K, B, K, nthreads=cores)
     dt = time.perf_counter() - t0
     tf = 2.0 * M * ncols * K / dt / 1e12
-    # reference CPU case: fp32 NN 480x512x512, U[-1,1] seed 0, one core, median of 3
+    # 4096^3 on one core: 128 columns, scaled to 4096
+    n1 = 128
+    t0 = time.perf_counter()
+    oracle.gemm_f32("TN", M, n1, K, 1.0, A, K, B[: K * n1], K)
+    dt1 = time.perf_counter() - t0
+    full_1core_s = dt1 * (4096 / n1)
+    # reference CPU case: fp32 NN 480x512x512, U[-1,1] seed 0
     m, n, k = 480, 512, 512
     r0 = np.random.default_rng(0)
     a = r0.uniform(-1, 1, m * k).astype(np.float32)
     b = r0.uniform(-1, 1, k * n).astype(np.float32)
-    ts = []
-    for _ in range(3):
-        t1 = time.perf_counter()
-        oracle.gemm_f32("NN", m, n, k, 1.0, a, m, b, k)
-        ts.append(time.perf_counter() - t1)
-    t1 = sorted(ts)[1]
+
+    def med3(threads):
+        ts = []
+        for _ in range(3):
+            t1 = time.perf_counter()
+            oracle.gemm_f32("NN", m, n, k, 1.0, a, m, b, k, nthreads=threads)
+            ts.append(time.perf_counter() - t1)
+        return sorted(ts)[1]
+
+    c1 = med3(1)
+    call = med3(cores)
+    fl1 = 2.0 * m * n * k
     return {"value": round(tf, 6), "unit": "TFLOP/s", "cores": cores, "kind": "port",
             "sample": f"4096x{ncols}x4096 fp32 TN ({ncols} of 4096 columns of the headline GEMM), "
                       f"{dt:.2f} s on {cores} threads",
             "gpu_speedup": round((2.0 * 4096 ** 3 / (gpu_ms * 1e-3) / 1e12) / tf, 1),
-            "config1_480x512x512_1core_s": round(t1, 4),
-            "config1_gflops_1core": round(2.0 * m * n * k / t1 / 1e9, 3)}
+            "gemm_4096_1core": {"sample": f"{n1} of 4096 columns, {dt1:.2f} s", "extrapolated_s": round(full_1core_s, 1),
+                                "gflops": round(2.0 * M * n1 * K / dt1 / 1e9, 3)},
+            "config1_480x512x512": {"1core_s": round(c1, 4), "1core_gflops": round(fl1 / c1 / 1e9, 3),
+                                    "allcores_s": round(call, 4), "allcores_gflops": round(fl1 / call / 1e9, 3),
+                                    "allcores": cores},
+            "compiler": "gcc -O2 -ffp-contract=off -fno-fast-math (oracle/Makefile)"}
 
 
 if __name__ == "__main__":

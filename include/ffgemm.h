@@ -108,6 +108,22 @@ const char *ffgemm_version(void);
  * (src/gemmMfmaKernelWriter.h:321-405). */
 int ffgemm_query_tile(const ffgemm_problem *p, ffgemm_tile *out);
 
+/* Maps the reference's tuning record onto a tile record that runs problem p:
+ * MFMA tiles per wave r, s (-r -s), waves x, y (-x -y), MFMA width z (-z, 16
+ * or 32), DepthU (-u) and LDS buffers (-f), i.e. MT0 = z*r*x, MT1 = z*s*y
+ * (src/gemmMfma.cpp:30-66, src/gemmMfmaKernelWriter.h:11-22).  Returns
+ * SUCCESS when a kernel has exactly that record; RTN_WARN when it runs as the
+ * nearest supported record instead: the same macro tile with the engine's
+ * own wave tiling, DepthU and LDS stages, or else the smallest macro tile
+ * covering MT0 x MT1 (256 x 256 when none does) with 32 x 32 MFMAs.  *out
+ * receives the record that runs, as ffgemm_query_tile reports it; the
+ * message (ffgemm_last_error) names what changed.  RTN_ERR: a record out of
+ * range, or a problem that cannot run.  Where the reference refuses a record
+ * for this problem (checkKernelParameters, src/gemmMfmaKernelWriter.h:
+ * 392-399), this build still runs it: the supported records are a superset. */
+int ffgemm_tile_from_reference(const ffgemm_problem *p, int r, int s, int x, int y, int z, int depth_u,
+                               int lds_buffers, ffgemm_tile *out);
+
 /* Loads the tuning table that problems run without a tile consult: the
  * autotuner's best tile per (dtype, layout, M, N, K), one CSV line each,
  * `dtype,layout,M,N,K,mt_m,mt_n,group_m,waves_m,waves_n,mfma_mn,tflops` (a line
@@ -184,7 +200,10 @@ int ffgemm_fp8_quantize(const float *x, int64_t rows, int64_t k, int64_t ldx, in
 int ffgemm_convert(const float *src, void *dst, int64_t n, int dtype_dst, void *stream);
 
 /* Grouped GEMM: `count` independent problems (same dtype, layout, compute)
- * run by one persistent launch.  A plan uploads the problem descriptors and
+ * run by one persistent launch.  The grouped kernels are 16-bit: bf16 or fp16
+ * operands with D in the same type and the default compute; any other
+ * problem (fp32, bf16x3, fp8, fp64, a different output type) or batch > 1 is
+ * refused by ffgemm_grouped_plan_create with RTN_ERR, before it allocates.  A plan uploads the problem descriptors and
  * the tile schedule (every output tile dealt to a workgroup of the grid,
  * longest K first) once; this call allocates device memory and synchronises.
  * After it, ffgemm_grouped_plan_run only launches a kernel and can be

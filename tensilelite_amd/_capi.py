@@ -63,6 +63,8 @@ def lib() -> ctypes.CDLL:
     L.ffgemm_version.restype = ctypes.c_char_p
     L.ffgemm_device_cu_count.restype = i32
     L.ffgemm_query_tile.argtypes = [PP, PT]; L.ffgemm_query_tile.restype = i32
+    L.ffgemm_tile_from_reference.argtypes = [PP] + [i32] * 7 + [PT]
+    L.ffgemm_tile_from_reference.restype = i32
     L.ffgemm_tune_db_load.argtypes = [ctypes.c_char_p, ctypes.POINTER(i32)]; L.ffgemm_tune_db_load.restype = i32
     L.ffgemm_run.argtypes = [PP, vp, vp, vp, vp, PT, vp]; L.ffgemm_run.restype = i32
     L.ffgemm_workspace_size.argtypes = [PP, ctypes.POINTER(ctypes.c_size_t)]

@@ -498,6 +498,59 @@ int ffgemm_query_tile(const ffgemm_problem* p, ffgemm_tile* out) {
     return FFGEMM_SUCCESS;
 }
 
+int ffgemm_tile_from_reference(const ffgemm_problem* p, int r, int s, int x, int y, int z, int depth_u,
+                               int lds_buffers, ffgemm_tile* out) {
+    if (!p || !out) return fail(FFGEMM_ERR, "null pointer");
+    if (r <= 0 || s <= 0 || x <= 0 || y <= 0 || (z != 16 && z != 32))
+        return fail(FFGEMM_ERR, "tuning record r=%d s=%d x=%d y=%d z=%d out of range", r, s, x, y, z);
+    // MT0 = z*r*x, MT1 = z*s*y (src/gemmMfma.cpp:53); 64-bit so large factors
+    // cannot overflow before the comparison below
+    const int64_t mt0 = (int64_t)z * r * x, mt1 = (int64_t)z * s * y;
+    const int64_t waves = (int64_t)x * y;
+    // the exact record, when a kernel has it
+    ffgemm_tile t;
+    memset(&t, 0, sizeof(t));
+    bool exact = false;
+    if (tile_supported((int)std::min<int64_t>(mt0, 1 << 20), (int)std::min<int64_t>(mt1, 1 << 20))) {
+        t.mt_m = (int)mt0; t.mt_n = (int)mt1;
+        t.mfma_mn = z;
+        if (x == 2 && (y == 2 || y == 4)) { t.waves_m = 2; t.waves_n = y; }
+        ffgemm_tile q = t;
+        exact = ffgemm_query_tile(p, &q) == FFGEMM_SUCCESS;
+        if (!exact && t.waves_m) {  // the shape exists, the wave tiling not: the tile's own engine
+            t.waves_m = t.waves_n = 0;
+            q = t;
+            exact = ffgemm_query_tile(p, &q) == FFGEMM_SUCCESS;
+        }
+        if (exact) t = q;
+    }
+    if (!exact) {
+        // the smallest macro tile covering the record's (largest when none covers
+        // it), 32 x 32 MFMAs, its default engine
+        int best = -1;
+        for (int i = 0; i < 5; ++i) {
+            if (kTiles[i].m < mt0 || kTiles[i].n < mt1) continue;
+            if (best < 0 || kTiles[i].m * kTiles[i].n < kTiles[best].m * kTiles[best].n) best = i;
+        }
+        if (best < 0) best = 0;
+        memset(&t, 0, sizeof(t));
+        t.mt_m = kTiles[best].m; t.mt_n = kTiles[best].n;
+        ffgemm_tile q = t;
+        const int rc = ffgemm_query_tile(p, &q);
+        if (rc != FFGEMM_SUCCESS) return rc;
+        t = q;
+    }
+    *out = t;
+    const bool same = exact && (depth_u <= 0 || depth_u == t.depth_u) &&
+                      (lds_buffers <= 0 || lds_buffers == t.lds_buffers) &&
+                      (waves == (int64_t)t.waves_m * t.waves_n);
+    if (same) return FFGEMM_SUCCESS;
+    return fail(FFGEMM_WARN, "record MT%lldx%lld (z=%d, %lld waves, depth_u %d, %d LDS buffers) runs as MT%dx%d "
+                "(z=%d, %dx%d waves, depth_u %d, %d LDS stages)", (long long)mt0, (long long)mt1, z,
+                (long long)waves, depth_u, lds_buffers, t.mt_m, t.mt_n, t.mfma_mn, t.waves_m, t.waves_n,
+                t.depth_u, t.lds_buffers);
+}
+
 int ffgemm_run(const ffgemm_problem* p, const void* A, const void* B, const void* C, void* D,
                const ffgemm_tile* tile, void* stream) {
     if (p && p->dtype == FFGEMM_F64) return run_f64(p, A, B, C, D, (hipStream_t)stream);
@@ -753,6 +806,12 @@ int ffgemm_grouped_plan_create(ffgemm_grouped_plan* plan, int count, const ffgem
         if (i == 0) { mode = L.mode; out = L.out; layout = L.layout; }
         else if (L.mode != mode || L.out != out || L.layout != layout)
             return fail(FFGEMM_ERR, "grouped problems must share dtype, compute and layout");
+        // the persistent kernels exist for bf16 -> bf16 and fp16 -> fp16 only;
+        // refuse anything else here, not at ffgemm_grouped_plan_run (which a
+        // caller may be capturing into a graph)
+        if (!((L.mode == kBF16 && L.out == kOutBF16) || (L.mode == kF16 && L.out == kOutF16)))
+            return fail(FFGEMM_ERR, "grouped GEMM runs bf16 or fp16 problems with the output in the input type "
+                        "(problem %d: dtype %d, output %d, compute %d)", i, p.dtype, p.dtype_out, p.compute);
         descs[i].a = L.args;  // group height as the single GEMM's default (prepare)
     }
     // Tile schedule, longest-processing-time first: tiles in K-descending

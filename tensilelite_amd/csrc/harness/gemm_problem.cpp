@@ -140,16 +140,16 @@ protected:
             explicit_tile_ = true;
         } else if (ca->Given("mfma-pttn0") || ca->Given("mfma-pttn1") || ca->Given("wave-pttn0") ||
                    ca->Given("wave-pttn1") || ca->Given("mfma-mn")) {
-            int z = (int)ca->Int("mfma-mn");
-            tile_.mt_m = z * (int)ca->Int("mfma-pttn0") * (int)ca->Int("wave-pttn0");
-            tile_.mt_n = z * (int)ca->Int("mfma-pttn1") * (int)ca->Int("wave-pttn1");
-            // -z 16 asks for 16 x 16 x 32 MFMAs (the tile record's MFMA knob)
-            if (ca->Given("mfma-mn")) tile_.mfma_mn = z;
-            // -x -y are the waves per group: the wave tiling of the tile record
-            if (ca->Given("wave-pttn0") || ca->Given("wave-pttn1")) {
-                tile_.waves_m = (int)ca->Int("wave-pttn0");
-                tile_.waves_n = (int)ca->Int("wave-pttn1");
-            }
+            // the reference's record: MT0 = z*r*x, MT1 = z*s*y, x*y waves, -u
+            // DepthU, -f LDS buffers; a record no kernel has exactly runs as the
+            // nearest supported one (stated as a warning)
+            const int st = ffgemm_tile_from_reference(
+                &prob_, (int)ca->Int("mfma-pttn0"), (int)ca->Int("mfma-pttn1"), (int)ca->Int("wave-pttn0"),
+                (int)ca->Int("wave-pttn1"), (int)ca->Int("mfma-mn"),
+                ca->Given("depth-u") ? (int)ca->Int("depth-u") : 0, ca->Given("buffer") ? (int)ca->Int("buffer") : 0,
+                &tile_);
+            if (st == FFGEMM_WARN) FF_WARN("%s", ffgemm_last_error());
+            else if (st != FFGEMM_SUCCESS) FF_ERR("tuning record: %s", ffgemm_last_error());
             explicit_tile_ = true;
         }
         int st = ffgemm_query_tile(&prob_, &tile_);
@@ -201,9 +201,11 @@ protected:
     ~GemmMfmaSolution() override { RuntimeHip::GetInstance()->DevFree(ws_); }
 
     // src/gemmMfma.cpp:348-421 with these tolerances instead of |d| > 0.1: fp32
-    // relative Frobenius <= 1e-5 (fp8 1e-1, its quantisation included); 16-bit
-    // per element |d| <= 1e-2 (|ref| + rms).  Returns RTN_WARN on a mismatch so
-    // the process exit code reflects correctness.
+    // relative Frobenius <= 1e-5; fp8 <= 5e-5 (the reference product is computed
+    // from the dequantised operands, so quantisation is not in this error: it
+    // measures the GEMM's fp32 accumulation over products up to 448^2 in
+    // magnitude); 16-bit per element |d| <= 1e-2 (|ref| + rms).  Returns
+    // RTN_WARN on a mismatch so the process exit code reflects correctness.
     E_ReturnState verifyResult() override {
         if (!g.verify) return E_ReturnState::SUCCESS;
         const size_t nD = hRef.size();
@@ -242,7 +244,7 @@ protected:
         const double rel = den > 0 ? std::sqrt(num / den) : std::sqrt(num);
         FF_LOG("verify: rel Frobenius error %.3e, %lld elements beyond 1e-2*(|ref|+rms), %lld beyond |0.1|", rel,
                (long long)bad, (long long)abs_bad);
-        const bool ok = (g.dtype == 1 || g.dtype == 4) ? (rel <= (g.dtype == 4 ? 1e-1 : 1e-5)) : bad == 0;
+        const bool ok = (g.dtype == 1 || g.dtype == 4) ? (rel <= (g.dtype == 4 ? 5e-5 : 1e-5)) : bad == 0;
         if (ok) {
             FF_LOG("Verify Success.");
             return E_ReturnState::SUCCESS;

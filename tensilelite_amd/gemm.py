@@ -124,6 +124,21 @@ def query_tile(problem: C.Problem, tile=None) -> C.Tile:
     return t
 
 
+def tile_from_reference(problem: C.Problem, r: int, s: int, x: int, y: int, z: int = 32, depth_u: int = 0,
+                        lds_buffers: int = 0):
+    """The tile record that runs the reference's tuning record (-r -s -x -y -z
+    -u -f) on ``problem``: (status, Tile, message).  status is SUCCESS when a
+    kernel has that exact record, WARN when it runs as the nearest supported
+    one; RTN_ERR raises."""
+    t = C.Tile()
+    st = C.lib().ffgemm_tile_from_reference(ctypes.byref(problem), r, s, x, y, z, depth_u, lds_buffers,
+                                            ctypes.byref(t))
+    if st not in (C.SUCCESS, C.WARN):
+        C.check(st)
+    msg = C.lib().ffgemm_last_error().decode() if st == C.WARN else ""
+    return st, t, msg
+
+
 def gemm(layout, M, N, K, A, B, Cm=None, D=None, *, lda=None, ldb=None, ldc=None, ldd=None,
          alpha=1.0, beta=0.0, batch=1, bsA=0, bsB=0, bsC=0, bsD=0, compute=C.COMPUTE_DEFAULT,
          round=C.ROUND_RNE, dtype_out=None, scaleA=None, scaleB=None, tile=None, stream=None,

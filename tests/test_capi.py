@@ -337,3 +337,94 @@ def test_committed_tune_table_is_valid():
                 assert t.group_m == gm, k
     finally:
         C.check(C.lib().ffgemm_tune_db_load(b"", None))
+
+
+def _support_matrix():
+    import json
+    path = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "support_matrix.json")
+    return json.load(open(path))
+
+
+def test_support_matrix_fixture_matches_reference_script():
+    """The committed support table holds the 32 records of test/gemm_test.sh:
+    5-73 (r, s, x, y in {1, 2}, u in {16, 32}) on M=768, N=640, K=256, with the
+    reference's Y/N verdicts: every record for fp16, 22 of 32 for fp32."""
+    d = _support_matrix()["dtypes"]
+    want = {(r, s, x, y, u) for r in (1, 2) for s in (1, 2) for x in (1, 2) for y in (1, 2) for u in (16, 32)}
+    for dt, n_yes in (("1", 22), ("2", 32)):
+        assert d[dt]["problem"] == {"M": 768, "N": 640, "K": 256}
+        cfgs = d[dt]["configs"]
+        assert {(c["r"], c["s"], c["x"], c["y"], c["u"]) for c in cfgs} == want
+        assert sum(c["support"] == "Y" for c in cfgs) == n_yes
+        assert all(c["reason"] for c in cfgs if c["support"] == "N")
+
+
+@pytest.mark.parametrize("dtype", [C.F32, C.F16, C.BF16])
+def test_tile_from_reference_runs_every_support_matrix_record(dtype):
+    """Every record of the reference's support matrix maps to a tile record
+    that the library accepts for the matrix's problem (a superset of the
+    reference's Y set: its N records run too), as SUCCESS when exact and as
+    RTN_WARN naming what changed otherwise."""
+    d = _support_matrix()["dtypes"]["1" if dtype == C.F32 else "2"]
+    M, N, K = d["problem"]["M"], d["problem"]["N"], d["problem"]["K"]
+    p = _p("TN", dtype, M, N, K)
+    for c in d["configs"]:
+        st, t, msg = tl.tile_from_reference(p, c["r"], c["s"], c["x"], c["y"], 32, c["u"], 0)
+        assert st in (C.SUCCESS, C.WARN), c
+        assert (st == C.WARN) == bool(msg)
+        mt0, mt1 = 32 * c["r"] * c["x"], 32 * c["s"] * c["y"]
+        assert t.mt_m >= mt0 and t.mt_n >= mt1, (c, t.mt_m, t.mt_n)
+        q = tl.query_tile(p, t)
+        assert (q.mt_m, q.mt_n, q.depth_u) == (t.mt_m, t.mt_n, t.depth_u)
+
+
+def test_tile_from_reference_exact_and_nearest():
+    p = _p("TN", C.BF16, 4096, 4096, 4096)
+    # 256 x 256 with 2 x 4 waves: the eight-wave engine, exactly
+    st, t, _ = tl.tile_from_reference(p, 4, 2, 2, 4, 32)
+    assert st == C.SUCCESS and (t.mt_m, t.mt_n, t.waves_m, t.waves_n) == (256, 256, 2, 4)
+    # 128 x 128 with 2 x 2 waves and DepthU 32 (the pipelined engine's 64-byte K rows): exact
+    st, t, _ = tl.tile_from_reference(p, 2, 2, 2, 2, 32, 32)
+    assert st == C.SUCCESS and (t.mt_m, t.mt_n, t.depth_u) == (128, 128, 32)
+    # ... with DepthU 16: the same tile, flagged
+    st, t, msg = tl.tile_from_reference(p, 2, 2, 2, 2, 32, 16)
+    assert st == C.WARN and (t.mt_m, t.mt_n) == (128, 128) and "depth_u 16" in msg
+    # 128 x 64 has no kernel: the smallest covering tile, 128 x 128
+    st, t, msg = tl.tile_from_reference(p, 2, 1, 2, 2, 32)
+    assert st == C.WARN and (t.mt_m, t.mt_n) == (128, 128) and "MT128x64" in msg
+    # 32 x 32 -> 64 x 64; 16 x 16 MFMAs at 64 x 64 (z=16, r=s=2, x=y=2): exact
+    st, t, _ = tl.tile_from_reference(p, 1, 1, 1, 1, 32)
+    assert st == C.WARN and (t.mt_m, t.mt_n) == (64, 64)
+    st, t, _ = tl.tile_from_reference(p, 2, 2, 2, 2, 16)
+    assert st == C.SUCCESS and (t.mt_m, t.mt_n, t.mfma_mn) == (64, 64, 16)
+    # larger than any tile: 256 x 256
+    st, t, _ = tl.tile_from_reference(p, 8, 8, 2, 2, 32)
+    assert st == C.WARN and (t.mt_m, t.mt_n) == (256, 256)
+    with pytest.raises(tl.FFGemmError):
+        tl.tile_from_reference(p, 0, 1, 1, 1, 32)
+    with pytest.raises(tl.FFGemmError):
+        tl.tile_from_reference(p, 1, 1, 1, 1, 8)
+
+
+@pytest.mark.parametrize("kind", ["f32", "bf16x3", "fp8", "bf16_to_f32", "f64"])
+def test_grouped_plan_create_rejects_non_16bit_groups(kind):
+    """The grouped kernels run bf16 -> bf16 and fp16 -> fp16 only: any other
+    group is refused by ffgemm_grouped_plan_create itself (before it
+    allocates), not later at ffgemm_grouped_plan_run, which a caller may be
+    capturing into a graph.  No device memory is touched: the pointers are
+    placeholders that are never dereferenced."""
+    M = N = K = 256
+    dt = {"f32": C.F32, "bf16x3": C.F32, "fp8": C.F8E4M3, "bf16_to_f32": C.BF16, "f64": C.F64}[kind]
+    layout = "TN" if kind == "fp8" else "NN"
+    p = _p(layout, dt, M, N, K, compute=C.COMPUTE_BF16X3 if kind == "bf16x3" else 0,
+           dtype_out=C.F32 if kind == "bf16_to_f32" else 0)
+    probs = (C.Problem * 2)(p, p)
+    fake = [ctypes.c_void_p(0x100000 * (i + 1)) for i in range(3)]
+    arr = lambda v: (ctypes.c_void_p * 2)(v, v)
+    plan = ctypes.c_void_p(0)
+    st = C.lib().ffgemm_grouped_plan_create(ctypes.byref(plan), 2, probs, arr(fake[0]), arr(fake[1]), None,
+                                            arr(fake[2]), None)
+    assert st == C.ERR
+    assert plan.value is None
+    msg = C.lib().ffgemm_last_error().decode()
+    assert ("bf16 or fp16" in msg) or ("dtype" in msg), msg

@@ -110,6 +110,59 @@ def test_fp32_accuracy_forms(cuda, layout, tile, compute):
     assert err <= 1e-5
 
 
+@pytest.mark.parametrize("layout", LAYOUTS)
+@pytest.mark.parametrize("tile", [(128, 256), (256, 128)])
+def test_bf16x3_regsplit_rect_tiles(cuda, layout, tile):
+    """The in-register bf16x3 split on the rectangular tiles, with M and N
+    past the first 128 rows / columns of a tile (both halves of a 256 side
+    valid, ragged ends), padded leading dimensions and alpha / beta with C."""
+    M, N, K = 300, 520, 200
+    _say(f"case bf16x3_regsplit {layout} tile={tile}")
+    rng = np.random.default_rng(13)
+    (ra, ca), (rb, cb) = op_dims(layout, M, N, K)
+    lda, ldb, ldc = ra + 4, rb + 8, M + 4
+    A = colmajor(rng, ra, ca, lda, kind="normal")
+    B = colmajor(rng, rb, cb, ldb, kind="normal")
+    Cm = colmajor(rng, M, N, ldc, kind="normal")
+    D = tl.gemm(layout, M, N, K, torch.from_numpy(A).to(cuda), torch.from_numpy(B).to(cuda),
+                torch.from_numpy(Cm).to(cuda), lda=lda, ldb=ldb, ldc=ldc, ldd=ldc, alpha=0.75, beta=-0.5,
+                compute=C.COMPUTE_BF16X3, tile=tile, workspace=None)
+    torch.cuda.synchronize()
+    ref = _f64_ref(layout, M, N, K, A, lda, B, ldb, 0.75, -0.5, Cm, ldc, ldc)
+    got = D.cpu().numpy()
+    err = rel_fro(extract(got, M, N, ldc), extract(ref, M, N, ldc))
+    _say(f"rel_fro={err:.3e}")
+    assert err <= 1e-5
+
+
+@pytest.mark.parametrize("layout", ["NN", "TN"])
+def test_bf16x3_regsplit_default_tile_4096(cuda, layout):
+    """bf16x3 with neither a workspace nor a tile at 4096^3, where the size
+    pick says 256 x 256: the library runs the in-register split on 128 x 256
+    instead (that kernel spills at 256 x 256), which ffgemm_query_tile reports;
+    8 sampled output columns within 1e-5 of FP64 (normal(0, 1), seed 2)."""
+    S = 4096
+    _say(f"case bf16x3 regsplit default tile {layout} 4096")
+    p = tl.make_problem(layout, C.F32, S, S, S, S, S, S, S, compute=C.COMPUTE_BF16X3)
+    t = tl.query_tile(p)
+    assert (t.mt_m, t.mt_n) == (128, 256)
+    g = torch.Generator(device="cpu").manual_seed(2)
+    A = torch.randn(S * S, generator=g)
+    B = torch.randn(S * S, generator=g)
+    D = torch.empty(S * S, device=cuda)
+    tl.run(p, A.to(cuda), B.to(cuda), None, D, workspace=None)
+    torch.cuda.synchronize()
+    cols = np.random.default_rng(3).choice(S, 8, replace=False)
+    Bc = B.numpy().reshape(S, S)[cols].astype(np.float64)                 # column n of B (K-contiguous)
+    Am = A.numpy().reshape(S, S)
+    At = (Am.T if layout == "NN" else Am).astype(np.float64).ravel()      # op(A) with K contiguous
+    ref = oracle.gemm_f64("TN", S, len(cols), S, 1.0, At, S, Bc.ravel(), S).reshape(len(cols), S)
+    got = D.cpu().numpy().reshape(S, S)[cols]
+    err = rel_fro(got, ref)
+    _say(f"rel_fro={err:.3e}")
+    assert err <= 1e-5
+
+
 @pytest.mark.parametrize("name", ["bf16", "f16", "f32"])
 def test_alpha_beta_padding(cuda, name):
     M, N, K = 256, 192, 128
@@ -488,18 +541,21 @@ def test_fp64_oracle_product(cuda, layout):
     assert rel_fro(D.cpu().numpy(), ref) <= 1e-12
 
 
+@pytest.mark.parametrize("name", ["bf16", "f16"])
 @pytest.mark.parametrize("layout", ["NN", "NT", "TN"])
 @pytest.mark.parametrize("beta", [0.0, 1.0])
-def test_4096_bf16_sampled_columns(cuda, layout, beta):
-    """Config 2 at full size: 24 random output columns against the FP64 oracle
-    and every column sum against (1^T op(A)) op(B) in FP64."""
+def test_4096_sampled_columns(cuda, name, layout, beta):
+    """Config 2 at full size, bf16 and fp16 (output in the input type): 24
+    random output columns against the FP64 oracle and every column sum against
+    (1^T op(A)) op(B) + beta * (1^T C) in FP64."""
     M = N = K = 4096
-    _say(f"case 4096 {layout} beta={beta}")
+    _say(f"case 4096 {name} {layout} beta={beta}")
+    tdt = torch.bfloat16 if name == "bf16" else torch.float16
     g = torch.Generator(device="cpu").manual_seed(1)
     (ra, ca), (rb, cb) = op_dims(layout, M, N, K)
-    A32 = (torch.randn(ra * ca, generator=g) / np.sqrt(K)).to(torch.bfloat16)
-    B32 = (torch.randn(rb * cb, generator=g) / np.sqrt(K)).to(torch.bfloat16)
-    C32 = (torch.rand(M * N, generator=g) * 2 - 1).to(torch.bfloat16)
+    A32 = (torch.randn(ra * ca, generator=g) / np.sqrt(K)).to(tdt)
+    B32 = (torch.randn(rb * cb, generator=g) / np.sqrt(K)).to(tdt)
+    C32 = (torch.rand(M * N, generator=g) * 2 - 1).to(tdt)
     D = tl.gemm(layout, M, N, K, A32.to(cuda), B32.to(cuda), C32.to(cuda), beta=beta)
     torch.cuda.synchronize()
     Dh = D.float().cpu().numpy().reshape(N, M)
@@ -555,6 +611,53 @@ def test_fp8_4096_sampled_columns(cuda):
     ref = oracle.gemm_f64("TN", S, len(cols), S, 1.0, deq_a, S, deq_b, S).reshape(len(cols), S)
     got = D.cpu().numpy().reshape(S, S)[cols]
     assert rel_fro(got, ref) <= 5e-5
+
+
+def test_fp8_8192_sampled_columns(cuda):
+    """Config 3's FP8 form at full size: A, B ~ normal(0, 1) from seed 2 (fp32,
+    M-contiguous A and K-contiguous B as the NN precision problem stores them),
+    quantised on the device with per-tensor amax scales, fp32 out.  16 sampled
+    output columns against the FP64 product of the dequantised operands
+    (rel-Frobenius <= 5e-5: the GEMM's own error), and the whole form's error
+    against the FP64 product of the unquantised inputs, which is the e4m3
+    quantisation's (a few %)."""
+    S = 8192
+    _say("case fp8 8192 sampled")
+    g = torch.Generator(device="cpu").manual_seed(2)
+    A = torch.randn(S * S, generator=g).to(cuda)   # M-contiguous: (m, k) at m + k*S
+    B = torch.randn(S * S, generator=g).to(cuda)   # K-contiguous: (k, n) at k + n*S
+    qa, sa = tl.fp8.quantize(A, rows=S, k=S, ldx=S, trans_x=False)
+    qb, sb = tl.fp8.quantize(B, rows=S, k=S, ldx=S, trans_x=True)
+    D = tl.gemm("TN", S, S, S, qa, qb, lda=S, ldb=S, scaleA=sa, scaleB=sb, dtype_out=torch.float32)
+    torch.cuda.synchronize()
+    cols = np.random.default_rng(5).choice(S, 16, replace=False)
+    # the quantised bytes are the oracle encoder's (the quantiser is bit-exact;
+    # checked here on the sampled operand rows too)
+    a_h = A.cpu().numpy()
+    qa_h = qa.cpu().numpy()
+    sa_h, sb_h = sa.item(), sb.item()
+    assert sa_h == np.float32(oracle.fp8_scale(np.abs(a_h).max()))
+    At = a_h.reshape(S, S).T          # (m, k), K-contiguous rows
+    del a_h
+    rows = np.random.default_rng(6).choice(S, 64, replace=False)
+    assert np.array_equal(qa_h.reshape(S, S)[rows], oracle.f32_to_e4m3_bits(np.ascontiguousarray(At[rows]).ravel(),
+                                                                          np.float32(sa_h)).reshape(64, S))
+    deq_a = oracle.e4m3_bits_to_f32(qa_h).astype(np.float64) / sa_h
+    del qa_h
+    qb_h = qb.cpu().numpy().reshape(S, S)[cols]
+    deq_b = oracle.e4m3_bits_to_f32(qb_h.ravel()).astype(np.float64) / sb_h
+    ref = oracle.gemm_f64("TN", S, len(cols), S, 1.0, deq_a, S, deq_b, S).reshape(len(cols), S)
+    del deq_a
+    got = D.cpu().numpy().reshape(S, S)[cols]
+    err = rel_fro(got, ref)
+    _say(f"rel_fro vs dequantised FP64 = {err:.3e}")
+    assert err <= 5e-5
+    b_cols = B.cpu().numpy().reshape(S, S)[cols].astype(np.float64)
+    ref_x = oracle.gemm_f64("TN", S, len(cols), S, 1.0, np.ascontiguousarray(At).astype(np.float64).ravel(), S,
+                            b_cols.ravel(), S).reshape(len(cols), S)
+    err_x = rel_fro(got, ref_x)
+    _say(f"rel_fro vs unquantised FP64 = {err_x:.3e}")
+    assert 0.01 < err_x < 0.1
 
 
 @pytest.mark.parametrize("compute", ["f32", "bf16x3"])

@@ -68,7 +68,7 @@ def test_converters_bit_exact_vs_reference_header():
     np.testing.assert_array_equal(oracle.f32_to_f16_bits(x), want)
     for v in x[:2000]:
         assert oracle.lib().orc_f32_to_bf16_trunc(float(v)) == (ref.ref_cvt_f32_to_bf16(float(v)) & 0xFFFF)
-    for h in range(0, 65536, 7):
+    for h in range(65536):
         a = np.float32(oracle.lib().orc_f16_to_f32_quirk(h))
         b = np.float32(ref.ref_cvt_f16_to_f32(h))
         assert a.view(np.uint32) == b.view(np.uint32) or (np.isnan(a) and np.isnan(b)), hex(h)
@@ -111,8 +111,17 @@ def test_split_bf16x3_properties():
     h = oracle.bf16_bits_to_f32(hi).astype(np.float64)
     l = oracle.bf16_bits_to_f32(lo).astype(np.float64)
     np.testing.assert_array_equal(hi, oracle.f32_to_bf16_bits(x))
-    # |x - hi - lo| <= 2^-17 |x|: two 8-bit significands cover 16+ bits
+    # |x - hi - lo| <= 2^-16 |x|: two 8-bit significands cover 16+ bits
     assert np.all(np.abs(x - h - l) <= np.abs(x) * 2.0 ** -16)
+    # lo is x - hi rounded to nearest even in bf16: x - hi is exact in fp32
+    # (Sterbenz), so torch's RNE bf16 cast of that fp32 difference pins every
+    # bit of lo.  A truncating lo meets the error bound above but fails here.
+    diff = (x.astype(np.float64) - h).astype(np.float32)
+    assert np.array_equal(diff.astype(np.float64), x.astype(np.float64) - h)
+    want_lo = torch.from_numpy(diff).to(torch.bfloat16).view(torch.int16).numpy().view(np.uint16)
+    np.testing.assert_array_equal(lo, want_lo)
+    trunc_lo = (diff.view(np.uint32) >> 16).astype(np.uint16)
+    assert np.count_nonzero(trunc_lo != want_lo) > x.size // 4   # the pin separates the two roundings
 
 
 # ---------------------------------------------------------------- e4m3
@@ -206,15 +215,27 @@ def test_gemm_f32_brute_force_tiny_and_mt():
 
 
 def test_gemm_planted_mistakes_are_caught():
-    """A transposed operand or a dropped beta term must fail the checks above:
-    plant them in the oracle call and confirm the torch reference disagrees."""
+    """A transposed operand, a dropped beta term or a wrong leading dimension
+    must fail the checks above: plant each in the oracle call and confirm the
+    torch float64 reference disagrees with it (while the correct call agrees)."""
     rng = np.random.default_rng(9)
     M = N = K = 8
-    A = rng.uniform(-1, 1, M * K).astype(np.float32)
+    lda = M + 4
+    A = rng.uniform(-1, 1, lda * K).astype(np.float32)
     B = rng.uniform(-1, 1, K * N).astype(np.float32)
     Cm = rng.uniform(-1, 1, M * N).astype(np.float32)
-    good = oracle.gemm_f64("NN", M, N, K, 1.0, A, M, B, K, 1.0, Cm, M, M)
-    wrong_layout = oracle.gemm_f64("TN", M, N, K, 1.0, A, M, B, K, 1.0, Cm, M, M)
-    no_beta = oracle.gemm_f64("NN", M, N, K, 1.0, A, M, B, K, 0.0, Cm, M, M)
-    assert np.max(np.abs(good - wrong_layout)) > 1e-3
-    assert np.max(np.abs(good - no_beta)) > 1e-3
+    opA = torch.from_numpy(A.astype(np.float64)).reshape(K, lda)[:, :M].T    # (M, K)
+    opB = torch.from_numpy(B.astype(np.float64)).reshape(N, K).T             # (K, N)
+    ref = (opA @ opB + torch.from_numpy(Cm.astype(np.float64)).reshape(N, M).T).numpy()
+
+    def dev(d):
+        return np.max(np.abs(d.reshape(N, M).T - ref))
+
+    good = oracle.gemm_f64("NN", M, N, K, 1.0, A, lda, B, K, 1.0, Cm, M, M)
+    assert dev(good) < 1e-12
+    wrong_layout = oracle.gemm_f64("TN", M, N, K, 1.0, A, lda, B, K, 1.0, Cm, M, M)
+    no_beta = oracle.gemm_f64("NN", M, N, K, 1.0, A, lda, B, K, 0.0, Cm, M, M)
+    wrong_ld = oracle.gemm_f64("NN", M, N, K, 1.0, A, M, B, K, 1.0, Cm, M, M)
+    assert dev(wrong_layout) > 1e-3
+    assert dev(no_beta) > 1e-3
+    assert dev(wrong_ld) > 1e-3
