@@ -100,4 +100,5 @@ def test_support_matrix_through_kernarg_entry(cuda, dtype):
             assert bad == 0, worst
         # padding rows of D are never written
         assert torch.isnan(D.view(N, ldd)[:, M:].float()).all()
-    assert len(seen) >= 3
+    # the records' macro tiles (32..128 per side) run on the 64 x 64 and 128 x 128 kernels
+    assert seen == {(64, 64), (128, 128)}
