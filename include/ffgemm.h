@@ -160,6 +160,25 @@ int ffgemm_workspace_size(const ffgemm_problem *p, size_t *bytes);
 int ffgemm_run_ws(const ffgemm_problem *p, const void *A, const void *B, const void *C, void *D,
                   const ffgemm_tile *tile, void *workspace, size_t workspace_bytes, void *stream);
 
+/* bf16x3 without any allocation: ffgemm_run_ws with the hi/lo planes in a
+ * device array inside the library instead of a caller workspace.  No
+ * counterpart in the reference: it is for callers that cannot allocate
+ * (code under hipGraph capture, tight serving loops, harnesses that forbid
+ * hipMalloc).  The array holds FFGEMM_NOALLOC_SCRATCH_BYTES (64 MiB) per
+ * device, from the library's load until the process ends; its address is
+ * looked up once per device (ffgemm_init, or the first call on a device).
+ * Only bf16x3 problems (F32 operands, COMPUTE_BF16X3); a problem whose planes,
+ * (M + N) * 3 * roundup(K, 64) * 2 bytes per batch (each operand's planes
+ * rounded up to 256 bytes), exceed the array is refused with RTN_ERR before
+ * anything runs.  Problems with no planes path run the in-register split,
+ * which needs no scratch.  Every call on one GPU uses the same array: calls
+ * that overlap on one GPU (other streams, other threads) race; calls ordered
+ * on one stream do not.  Allocates nothing, with the results of ffgemm_run_ws
+ * for the same problem and tile. */
+#define FFGEMM_NOALLOC_SCRATCH_BYTES (64u << 20)
+int ffgemm_run_x3_noalloc(const ffgemm_problem *p, const void *A, const void *B, const void *C, void *D,
+                          const ffgemm_tile *tile, void *stream);
+
 /* The reference kernel's argument list, in its kernarg order
  * D, C, A, B, alpha, beta, ldd, bsD, ldc, bsC, lda, bsA, ldb, bsB
  * (src/gemmMfma.cpp:106-119, src/gemmMfmaKernelWriter.h:323-338), TN layout,

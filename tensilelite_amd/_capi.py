@@ -70,6 +70,7 @@ def lib() -> ctypes.CDLL:
     L.ffgemm_workspace_size.argtypes = [PP, ctypes.POINTER(ctypes.c_size_t)]
     L.ffgemm_workspace_size.restype = i32
     L.ffgemm_run_ws.argtypes = [PP, vp, vp, vp, vp, PT, vp, ctypes.c_size_t, vp]; L.ffgemm_run_ws.restype = i32
+    L.ffgemm_run_x3_noalloc.argtypes = [PP, vp, vp, vp, vp, PT, vp]; L.ffgemm_run_x3_noalloc.restype = i32
     u32 = ctypes.c_uint32
     L.ffgemm_run_tn_kernarg.argtypes = [vp, vp, vp, vp, f32, f32] + [u32] * 12 + [i32, PT, vp]
     L.ffgemm_run_tensile_args.argtypes = [vp, ctypes.c_size_t, i32, PT, vp]; L.ffgemm_run_tensile_args.restype = i32

@@ -429,6 +429,37 @@ int run_f64(const ffgemm_problem* p, const void* A, const void* B, const void* C
 
 }  // namespace
 
+// Scratch of ffgemm_run_x3_noalloc: the bf16x3 hi/lo planes of one problem,
+// in a device array every GPU holds from module load until the process ends.
+// Host code never passes `scratch` itself to a kernel (on the host it names a
+// stand-in, not device memory): it uses the per-device address from
+// hipGetSymbolAddress below.
+constexpr size_t kNoAllocScratchBytes = (size_t)FFGEMM_NOALLOC_SCRATCH_BYTES;
+static __device__ __align__(256) unsigned char scratch[kNoAllocScratchBytes];
+
+namespace {
+std::mutex g_scratch_mu;
+void* g_scratch_ptr[64] = {};
+
+// Device address of `scratch` on the current device, looked up once per device.
+int scratch_ptr(void** out) {
+    int dev = 0;
+    hipError_t e = hipGetDevice(&dev);
+    if (e != hipSuccess) return hip_fail(e, "hipGetDevice");
+    if (dev < 0 || dev >= 64) return fail(FFGEMM_FATAL, "device %d out of range", dev);
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if (!g_scratch_ptr[dev]) {
+        void* p = nullptr;
+        e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess) return hip_fail(e, "hipGetSymbolAddress(scratch)");
+        if (!p) return fail(FFGEMM_FATAL, "hipGetSymbolAddress(scratch) returned null");
+        g_scratch_ptr[dev] = p;
+    }
+    *out = g_scratch_ptr[dev];
+    return FFGEMM_SUCCESS;
+}
+}  // namespace
+
 extern "C" {
 
 int ffgemm_init(void) {
@@ -436,7 +467,8 @@ int ffgemm_init(void) {
     hipError_t e = hipGetDevice(&dev);
     if (e != hipSuccess) return hip_fail(e, "hipGetDevice");
     cu_count();
-    return FFGEMM_SUCCESS;
+    void* p = nullptr;
+    return scratch_ptr(&p);
 }
 
 void ffgemm_shutdown(void) {}
@@ -644,6 +676,26 @@ int ffgemm_run_ws(const ffgemm_problem* p, const void* A, const void* B, const v
     q.bsA = p->M * 3 * kp;
     q.bsB = p->N * 3 * kp;
     return ffgemm_run(&q, wa, wb, C, D, tile, stream);
+}
+
+int ffgemm_run_x3_noalloc(const ffgemm_problem* p, const void* A, const void* B, const void* C, void* D,
+                          const ffgemm_tile* tile, void* stream) {
+    if (!p) return fail(FFGEMM_ERR, "null problem");
+    if (p->dtype != FFGEMM_F32 || p->compute != FFGEMM_COMPUTE_BF16X3)
+        return fail(FFGEMM_ERR, "ffgemm_run_x3_noalloc runs bf16x3 problems (fp32 operands, compute BF16X3)");
+    // problems without a planes path split in registers: no scratch either
+    if (!x3_planes_apply(p) || mn_k_chunk(p)) return ffgemm_run(p, A, B, C, D, tile, stream);
+    const int64_t need = x3_ws_bytes(p);
+    if (need > (int64_t)kNoAllocScratchBytes)
+        return fail(FFGEMM_ERR, "bf16x3 planes of this problem need %lld bytes; the no-allocation scratch holds %zu "
+                    "((M + N) * 3 * roundup(K, 64) * 2 * batch must fit; use ffgemm_run_ws with a workspace)",
+                    (long long)need, kNoAllocScratchBytes);
+    void* ws = nullptr;
+    const int r = scratch_ptr(&ws);
+    if (r != FFGEMM_SUCCESS) return r;
+    // the split kernels write every element of both planes (K padding as
+    // zeros), so nothing left by an earlier call is read
+    return ffgemm_run_ws(p, A, B, C, D, tile, ws, kNoAllocScratchBytes, stream);
 }
 
 int ffgemm_run_tn_kernarg(void* D, const void* C, const void* A, const void* B, float alpha, float beta,
