@@ -416,6 +416,20 @@ def run_extras(a, torch, np, tl, C, dev):
             del A, B, D, graph
         ex["small_480x512x512"] = res
 
+    # the reference's thin "phantom" shapes (one free dimension 480 / 512):
+    # fp32 / bf16 / fp16 TN with the library's own plan (split-K through the
+    # workspace path included), beside the fraction of peak the reference
+    # published on its own chip; rows [dtype, M, N, K, TFLOP/s, % of MI355X
+    # peak, tile, split_k, reference % of gfx908 peak]
+    if want("phantom"):
+        from tensilelite_amd import phantom
+        rows = phantom.bench(dev)
+        ex["phantom"] = {"columns": ["dtype", "M", "N", "K", "tflops", "pct_peak", "tile", "split_k",
+                                     "reference_pct_peak_gfx908"],
+                         "rows": [[r["dtype"], r["M"], r["N"], r["K"], r["tflops"], r["pct_peak"],
+                                   "%dx%d" % tuple(r["tile"]), r["split_k"], r["reference_pct_peak_gfx908"]]
+                                  for r in rows]}
+
     # grouped: 64 problems, one persistent launch, hipGraph replay
     if want("grouped_64"):
         from tensilelite_amd import grouped

@@ -94,6 +94,11 @@ typedef struct ffgemm_tile {
                        * (bf16/fp16 at 64x64 and 128x128; -z of the reference) */
     int32_t group_m;  /* tile-raster group height (0 = default) */
     int32_t lds_buffers; /* 2 */
+    int32_t split_k;  /* K slices (split-K), summed by a reduction pass: 0 = the
+                       * library's choice, 1 = none, S >= 2 = S slices.  Slices
+                       * need the fp32 partial planes in a workspace, so only
+                       * ffgemm_run_ws splits (single problems, not bf16x3);
+                       * ffgemm_run takes 0 or 1. */
 } ffgemm_tile;
 
 /* Library / device bring-up.  Optional: every entry point initialises lazily. */
@@ -126,8 +131,9 @@ int ffgemm_tile_from_reference(const ffgemm_problem *p, int r, int s, int x, int
 
 /* Loads the tuning table that problems run without a tile consult: the
  * autotuner's best tile per (dtype, layout, M, N, K), one CSV line each,
- * `dtype,layout,M,N,K,mt_m,mt_n,group_m,waves_m,waves_n,mfma_mn,tflops` (a line
- * of 11 fields has no mfma_mn: 32 x 32 MFMAs).  It
+ * `dtype,layout,M,N,K,mt_m,mt_n,group_m,waves_m,waves_n,mfma_mn,split_k,tflops`
+ * (a line of 12 fields has no split_k: no split; of 11 fields no mfma_mn
+ * either: 32 x 32 MFMAs; an entry's split_k applies on the workspace path).  It
  * replaces the table in use.  path NULL: the default, the file named by
  * FFGEMM_TUNE_DB, else tune_db.csv in the directory above the library's;
  * "" or a missing file: an empty table.  Without this call the default is
@@ -146,17 +152,27 @@ int ffgemm_run(const ffgemm_problem *p, const void *A, const void *B, const void
                const ffgemm_tile *tile, void *stream);
 
 /* Device workspace (bytes) that ffgemm_run_ws can use for this problem; 0
- * when the problem has no workspace path.  Today only bf16x3: with a
- * workspace the fp32 operands are split once into bf16 hi/lo planes
- * ((M + N) rows x 3*roundup(K, 64) bf16 per batch) and the three products run
- * as one bf16 MFMA GEMM over K' = 3*roundup(K, 64); without one they are
- * split in registers inside the GEMM.  No counterpart in the reference (its
- * precision forms stop at fp32). */
+ * when the problem has no workspace path.  Two uses, neither in the reference:
+ *  - bf16x3: the fp32 operands are split once into bf16 hi/lo planes
+ *    ((M + N) rows x 3*roundup(K, 64) bf16 per batch) and the three products
+ *    run as one bf16 MFMA GEMM over K' = 3*roundup(K, 64); without a
+ *    workspace they are split in registers inside the GEMM;
+ *  - split-K: a single problem with too few output tiles to fill the CUs
+ *    (by the library's cost model, or a tile record with split_k >= 2) cuts K
+ *    into S slices whose fp32 partial products (S planes of roundup(M, 4) x N)
+ *    are summed by a reduction pass; without a workspace it runs unsplit. */
 int ffgemm_workspace_size(const ffgemm_problem *p, size_t *bytes);
+
+/* The tile record ffgemm_run_ws follows for this problem given a workspace of
+ * ffgemm_workspace_size bytes, split_k included (ffgemm_query_tile reports
+ * the ffgemm_run plan, which never splits).  *out as input: a tile to
+ * complete, or zeros. */
+int ffgemm_query_tile_ws(const ffgemm_problem *p, ffgemm_tile *out);
 
 /* ffgemm_run with a caller-owned device workspace (256-byte aligned).  Falls
  * back to ffgemm_run when the workspace is NULL or smaller than
- * ffgemm_workspace_size reports.  Calls on one workspace must not overlap. */
+ * ffgemm_workspace_size reports (RTN_ERR instead when the tile record asks
+ * for split_k >= 2).  Calls on one workspace must not overlap. */
 int ffgemm_run_ws(const ffgemm_problem *p, const void *A, const void *B, const void *C, void *D,
                   const ffgemm_tile *tile, void *workspace, size_t workspace_bytes, void *stream);
 

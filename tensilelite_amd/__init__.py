@@ -7,8 +7,8 @@ plans, the autograd Function and the autotuner.
 """
 from . import _capi, fp8
 from ._capi import FFGemmError, Problem, Tile, check, lib
-from .gemm import (convert, gemm, make_problem, make_tile, mm, query_tile, run, tile_from_reference,
+from .gemm import (convert, gemm, make_problem, make_tile, mm, query_tile, query_tile_ws, run, tile_from_reference,
                    workspace_size)
 
 __all__ = ["fp8", "FFGemmError", "Problem", "Tile", "check", "lib", "convert", "gemm", "make_problem",
-           "make_tile", "mm", "query_tile", "run", "tile_from_reference", "workspace_size", "_capi"]
+           "make_tile", "mm", "query_tile", "query_tile_ws", "run", "tile_from_reference", "workspace_size", "_capi"]

@@ -37,7 +37,7 @@ class Tile(ctypes.Structure):
     _fields_ = [
         ("mt_m", ctypes.c_int32), ("mt_n", ctypes.c_int32), ("depth_u", ctypes.c_int32),
         ("waves_m", ctypes.c_int32), ("waves_n", ctypes.c_int32), ("mfma_mn", ctypes.c_int32),
-        ("group_m", ctypes.c_int32), ("lds_buffers", ctypes.c_int32),
+        ("group_m", ctypes.c_int32), ("lds_buffers", ctypes.c_int32), ("split_k", ctypes.c_int32),
     ]
 
     def as_tuple(self):
@@ -63,6 +63,7 @@ def lib() -> ctypes.CDLL:
     L.ffgemm_version.restype = ctypes.c_char_p
     L.ffgemm_device_cu_count.restype = i32
     L.ffgemm_query_tile.argtypes = [PP, PT]; L.ffgemm_query_tile.restype = i32
+    L.ffgemm_query_tile_ws.argtypes = [PP, PT]; L.ffgemm_query_tile_ws.restype = i32
     L.ffgemm_tile_from_reference.argtypes = [PP] + [i32] * 7 + [PT]
     L.ffgemm_tile_from_reference.restype = i32
     L.ffgemm_tune_db_load.argtypes = [ctypes.c_char_p, ctypes.POINTER(i32)]; L.ffgemm_tune_db_load.restype = i32

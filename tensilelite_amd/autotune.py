@@ -7,10 +7,12 @@ The reference sweeps padding, LDS buffers, MFMA size, MFMA tiles per wave
 are the macro tile (MT_M x MT_N), the wave tiling (for 256 x 256: eight waves
 of 128 x 64, or four of 128 x 128 -- which also fixes the engine, its LDS
 ring depth and DepthU), the MFMA instruction (32 x 32 x 16, or 16 x 16 x 32
-at 64 x 64 and 128 x 128) and the tile-raster group height.  Results are
-kept in a CSV table keyed by (dtype, layout, M, N, K), one line
-``dtype,layout,M,N,K,mt_m,mt_n,group_m,waves_m,waves_n,mfma_mn,tflops`` per
-problem.
+at 64 x 64 and 128 x 128), the tile-raster group height and, for problems
+with few output tiles, split-K (K cut into S slices whose fp32 partials a
+reduction pass sums; it runs with a workspace).  Results are kept in a CSV
+table keyed by (dtype, layout, M, N, K), one line
+``dtype,layout,M,N,K,mt_m,mt_n,group_m,waves_m,waves_n,mfma_mn,split_k,tflops``
+per problem.
 The C library reads the same table (ffgemm_tune_db_load): a problem run with
 no tile takes its entry.  ``best_tile`` looks an entry up from Python.
 """
@@ -30,7 +32,10 @@ from .gemm import make_problem, make_tile, run
 TILES = [(256, 256, 2, 4, 0), (256, 256, 2, 2, 0), (256, 128, 0, 0, 0), (128, 256, 0, 0, 0), (128, 128, 0, 0, 0),
          (128, 128, 0, 0, 16), (64, 64, 0, 0, 0), (64, 64, 0, 0, 16)]
 DB_PATH = os.environ.get("FFGEMM_TUNE_DB", os.path.join(os.path.dirname(os.path.abspath(__file__)), "tune_db.csv"))
-FIELDS = ["dtype", "layout", "M", "N", "K", "mt_m", "mt_n", "group_m", "waves_m", "waves_n", "mfma_mn", "tflops"]
+FIELDS = ["dtype", "layout", "M", "N", "K", "mt_m", "mt_n", "group_m", "waves_m", "waves_n", "mfma_mn", "split_k",
+          "tflops"]
+# split-K candidates for problems whose best unsplit tile leaves CUs idle
+SPLITS = (1, 2, 3, 4, 6, 8, 12, 16)
 # an entry is recorded only when it beats the library's own choice by this
 # much: run-to-run differences on one box reach ~3 %
 MARGIN = 0.05
@@ -58,7 +63,7 @@ def load_db(path=None):
             for r in csv.DictReader(f):
                 db[key(int(r["dtype"]), r["layout"], int(r["M"]), int(r["N"]), int(r["K"]))] = {
                     "tile": [int(r[k]) for k in ("mt_m", "mt_n", "group_m", "waves_m", "waves_n")]
-                    + [int(r.get("mfma_mn") or 0)],
+                    + [int(r.get("mfma_mn") or 0), int(r.get("split_k") or 0)],
                     "tflops": float(r["tflops"])}
     return db
 
@@ -74,7 +79,7 @@ def save_db(db, path=None):
             dt, lay, mnk = k.split(":")
             M, N, K = mnk.split("x")
             e = db[k]
-            tile = list(e["tile"]) + [0] * (6 - len(e["tile"]))
+            tile = list(e["tile"]) + [0] * (7 - len(e["tile"]))
             w.writerow([dt, lay, M, N, K] + tile + [round(e["tflops"], 1)])
     os.replace(tmp, path)
     C.check(C.lib().ffgemm_tune_db_load(path.encode(), None))
@@ -84,49 +89,64 @@ def key(dtype_code, layout, M, N, K):
     return f"{dtype_code}:{layout}:{M}x{N}x{K}"
 
 
-def tune_gemm(problem: C.Problem, A, B, Cm, D, tiles=TILES, group_ms=(0,), steps=10, rounds=2, verbose=False):
-    """Times every candidate tile for one problem in `rounds` interleaved rounds
-    (each keeps its best time, so no candidate pays alone for the clock ramp);
-    returns {'best': tile, 'results': [...]} fastest first."""
+def split_workspace_bytes(problem: C.Problem, S: int) -> int:
+    """Workspace of a split into S slices: S fp32 planes of roundup(M, 4) x N
+    (the C library's layout), rounded up to 256 bytes."""
+    return (S * ((problem.M + 3) // 4 * 4) * problem.N * 4 + 255) // 256 * 256
+
+
+def tune_gemm(problem: C.Problem, A, B, Cm, D, tiles=TILES, group_ms=(0,), splits=(1,), steps=10, rounds=2,
+              verbose=False):
+    """Times every candidate tile (x split-K count) for one problem in `rounds`
+    interleaved rounds (each keeps its best time, so no candidate pays alone
+    for the clock ramp); returns {'best': tile, 'results': [...]} fastest
+    first.  Every candidate runs through ffgemm_run_ws with one workspace
+    large enough for the largest split."""
     flop = 2.0 * problem.M * problem.N * problem.K * max(1, problem.batch)
+    nws = max(split_workspace_bytes(problem, S) for S in splits) if max(splits) > 1 else 0
+    ws = torch.empty(max(nws, 256), dtype=torch.uint8, device=torch.device("cuda", torch.cuda.current_device()))
     cands = []
     for t in tiles:
         wm, wn = (t[2], t[3]) if len(t) > 3 else (0, 0)
         mf = t[4] if len(t) > 4 else 0
         for gm in group_ms:
-            rec = (t[0], t[1], gm, wm, wn, mf)
-            try:
-                run(problem, A, B, Cm, D, make_tile(rec))
-            except C.FFGemmError:
-                continue  # no kernel for this tile / wave tiling with this dtype and layout
-            cands.append(rec)
+            for S in splits:
+                rec = (t[0], t[1], gm, wm, wn, mf, S)
+                try:
+                    run(problem, A, B, Cm, D, make_tile(rec), workspace=ws)
+                except C.FFGemmError:
+                    continue  # no kernel for this tile / wave tiling / split with this dtype and layout
+                cands.append(rec)
     best = {}
     for _ in range(rounds):
         for rec in cands:
             tile = make_tile(rec)
-            best[rec] = min(best.get(rec, float("inf")), _time(lambda: run(problem, A, B, Cm, D, tile), steps=steps))
+            best[rec] = min(best.get(rec, float("inf")),
+                            _time(lambda: run(problem, A, B, Cm, D, tile, workspace=ws), steps=steps))
     results = sorted(({"tile": list(rec), "ms": ms, "tflops": flop / (ms * 1e-3) / 1e12} for rec, ms in best.items()),
                      key=lambda r: r["ms"])
     if verbose:
         for r in results:
             t = r["tile"]
-            print(f"MT{t[0]}x{t[1]} waves={t[3]}x{t[4]} mfma={t[5] or 32} gm={t[2]}: "
+            print(f"MT{t[0]}x{t[1]} waves={t[3]}x{t[4]} mfma={t[5] or 32} gm={t[2]} split={t[6]}: "
                   f"Performance = {r['tflops']:.2f} TFLOPs")
     return {"best": tuple(results[0]["tile"]) if results else None, "results": results}
 
 
 def library_default(problem: C.Problem):
     """The candidate record of the tile the library picks with an empty table
-    (at 256 x 256 with its wave tiling, which the candidates name there)."""
+    and a workspace (at 256 x 256 with its wave tiling, which the candidates
+    name there; its split-K count)."""
     C.check(C.lib().ffgemm_tune_db_load(b"", None))
     try:
         t = C.Tile()
-        C.check(C.lib().ffgemm_query_tile(ctypes.byref(problem), ctypes.byref(t)))
+        C.check(C.lib().ffgemm_query_tile_ws(ctypes.byref(problem), ctypes.byref(t)))
     finally:
         C.check(C.lib().ffgemm_tune_db_load(None, None))
+    S = max(1, t.split_k)
     if (t.mt_m, t.mt_n) == (256, 256):
-        return (256, 256, 0, t.waves_m, t.waves_n, 0)
-    return (t.mt_m, t.mt_n, 0, 0, 0, 0)
+        return (256, 256, 0, t.waves_m, t.waves_n, 0, S)
+    return (t.mt_m, t.mt_n, 0, 0, 0, 0, S)
 
 
 def tune_and_record(problem, A, B, Cm, D, db=None, path=None, **kw):
@@ -215,13 +235,28 @@ def main(argv=None):
     ap.add_argument("--rounds", type=int, default=3)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--group-ms", default="0", help="tile-raster group heights to try, e.g. 0,2,8 (0: default)")
+    ap.add_argument("--set", default="bench", choices=("bench", "phantom", "all"),
+                    help="bench: the benchmark's problems and common shapes; phantom: the reference's thin "
+                         "shapes (test/phantom_test.sh) with split-K candidates")
     a = ap.parse_args(argv)
     gms = tuple(int(x) for x in a.group_ms.split(","))
     dev = torch.device("cuda", torch.cuda.current_device())
     db = load_db(a.db)
-    for (dt, lay, M, N, K, out) in TUNE_PROBLEMS:
+    todo = []
+    if a.set in ("bench", "all"):
+        todo += [(pr, (1,)) for pr in TUNE_PROBLEMS]
+    if a.set in ("phantom", "all"):
+        from . import phantom
+        code = {"f32": C.F32, "bf16": C.BF16, "f16": C.F16}
+        seen = set()
+        for dt, _, M, N, K, _ in phantom.problems():
+            if (dt, M, N, K) not in seen:
+                seen.add((dt, M, N, K))
+                todo.append(((code[dt], "TN", M, N, K, None), SPLITS))
+    for (dt, lay, M, N, K, out), splits in todo:
         p, A, B, D = _operands(dt, lay, M, N, K, out, dev)
-        r = tune_and_record(p, A, B, None, D, db=db, path=a.db, rounds=a.rounds, steps=a.steps, group_ms=gms)
+        r = tune_and_record(p, A, B, None, D, db=db, path=a.db, rounds=a.rounds, steps=a.steps, group_ms=gms,
+                            splits=splits)
         best = r["results"][0]
         ms_def = next((x["ms"] for x in r["results"] if tuple(x["tile"]) == r["default"]), float("nan"))
         rec = "recorded" if key(dt, lay, M, N, K) in db else "library default kept"

@@ -160,6 +160,59 @@ __global__ void convert_kernel(const float* __restrict__ src, void* __restrict__
     }
 }
 
+// Split-K reduction.  Thread t of the flattened grid takes 4 consecutive rows
+// m of one column n (the W slices are read as 16-byte vectors), sums the S
+// partials in slice order (deterministic), applies alpha / beta with C and
+// writes D in its type; columns whose last group of 4 passes M store element
+// by element.  HBM-bound: S * 16 bytes read per 4 outputs.
+__global__ void __launch_bounds__(256) splitk_reduce_kernel(SplitReduceArgs a) {
+    const int64_t mq = (a.M + 3) / 4;
+    const int64_t total = mq * a.N;
+    for (int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; t < total; t += (int64_t)gridDim.x * blockDim.x) {
+        const int n = (int)(t / mq);
+        const int m = (int)(t - (int64_t)n * mq) * 4;
+        const float* w = a.W + (int64_t)n * a.ldw + m;
+        f32x4_t s = *(const f32x4_t*)w;
+        for (int k = 1; k < a.S; ++k) {
+            const f32x4_t v = *(const f32x4_t*)(w + (int64_t)k * a.slice);
+            s[0] += v[0]; s[1] += v[1]; s[2] += v[2]; s[3] += v[3];
+        }
+        const int nv = min(4, a.M - m);
+        float r[4];
+#pragma unroll
+        for (int e = 0; e < 4; ++e) {
+            float c = 0.0f;
+            if (a.C && e < nv) {
+                const int64_t ci = (int64_t)n * a.ldc + m + e;
+                c = a.out == kOutF32 ? ((const float*)a.C)[ci]
+                  : a.out == kOutBF16 ? (float)((const __bf16*)a.C)[ci] : (float)((const _Float16*)a.C)[ci];
+            }
+            r[e] = a.C ? __builtin_fmaf(a.alpha, s[e], a.beta * c) : a.alpha * s[e];
+        }
+        const int64_t di = (int64_t)n * a.ldd + m;
+        if (a.out == kOutF32) {
+            float* d = (float*)a.D + di;
+            if (nv == 4) *(f32x4_t*)d = f32x4_t{r[0], r[1], r[2], r[3]};
+            else for (int e = 0; e < nv; ++e) d[e] = r[e];
+        } else {
+            uint32_t p0, p1;
+            if (a.out == kOutBF16) {
+                p0 = a.trunc ? pack_bf16x2_trunc(r[0], r[1]) : pack_bf16x2_rne(r[0], r[1]);
+                p1 = a.trunc ? pack_bf16x2_trunc(r[2], r[3]) : pack_bf16x2_rne(r[2], r[3]);
+            } else {
+                p0 = pack_f16x2(r[0], r[1]);
+                p1 = pack_f16x2(r[2], r[3]);
+            }
+            uint16_t* d = (uint16_t*)a.D + di;
+            if (nv == 4) *(i32x2_t*)d = i32x2_t{(int)p0, (int)p1};
+            else {
+                const uint16_t h[4] = {(uint16_t)p0, (uint16_t)(p0 >> 16), (uint16_t)p1, (uint16_t)(p1 >> 16)};
+                for (int e = 0; e < nv; ++e) d[e] = h[e];
+            }
+        }
+    }
+}
+
 // FP64 oracle GEMM on v_mfma_f64_16x16x4_f64: 64 x 64 tile per 256-thread
 // block, each wave a 32 x 32 quarter (2 x 2 MFMA tiles), K staged through LDS
 // 16 at a time.  Same column-major problem and layouts as the fp32 path.

@@ -25,6 +25,22 @@ __global__ void __launch_bounds__(256) x3_split_kernel(const float* __restrict__
                                                        int64_t ldx, int64_t bsx, int trans_x,
                                                        uint16_t* __restrict__ out, int64_t kp, int order);
 __global__ void convert_kernel(const float* __restrict__ src, void* __restrict__ dst, int64_t n, int to);
+
+// Split-K reduction: D = alpha * sum_s W[s] + beta * C, converted to D's type.
+// W holds the fp32 partial products of S K-slices, slice s at W + s * slice
+// (M x N, leading dimension ldw, a multiple of 4).
+struct SplitReduceArgs {
+    const float* W;
+    int64_t slice;
+    int32_t S, M, N, ldw;
+    const void* C;  // null when beta == 0
+    void* D;
+    int32_t ldc, ldd;
+    float alpha, beta;
+    int32_t out;    // OutKind
+    int32_t trunc;  // bf16 output by truncation
+};
+__global__ void __launch_bounds__(256) splitk_reduce_kernel(SplitReduceArgs a);
 __global__ void __launch_bounds__(256) gemm_f64_kernel(F64Args a);
 
 }  // namespace ffgemm

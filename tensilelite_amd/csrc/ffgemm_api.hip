@@ -111,6 +111,61 @@ Tile pick_tile(int64_t M, int64_t N, int64_t batch) {
     return best;
 }
 
+// ---------------------------------------------------------------------------
+// Split-K planning.  A problem with too few output tiles for the CUs can cut
+// K into S slices: one batched GEMM writes S fp32 partial planes to a
+// workspace and splitk_reduce_kernel sums them into D.
+// ---------------------------------------------------------------------------
+// Shortest slice: four 128-byte K-tiles.
+int64_t split_min_k(int mode) { return 4 * 128 / mode_elem_bytes(mode); }
+// Slice length: ceil(K / S) rounded up to 128 elements, which keeps every
+// slice's operand offset 16-byte aligned and whole K-tiles in all but the last.
+int64_t split_slice_k(int64_t K, int S) { return cdiv(cdiv(K, S), 128) * 128; }
+int64_t split_ldw(int64_t M) { return cdiv(M, 4) * 4; }
+
+// Modelled time (µs) of a launch plan on MI355X: passes of workgroups over the
+// CUs times one workgroup's time, i.e. its flop at 70 % of the mode's MFMA rate
+// at 1.9 GHz divided by the tile's relative efficiency (as pick_tile), plus
+// 4 µs of prologue and epilogue; for S > 1 also the reduction (S fp32 planes
+// read, D written, C read, at 4 TB/s) and 4 µs for its launch.
+double plan_cost(int64_t M, int64_t N, int64_t K, int mode, int osz, bool use_c, int ti, int S) {
+    static const double eff[] = {1.0, 0.82, 0.82, 0.70, 0.51};  // kTiles order
+    const double rate = (mode == kF8 ? 8192.0 : mode == kF32 ? 256.0 : 4096.0) * 1.9e3 * 0.7;  // flop/µs/CU
+    const Tile& t = kTiles[ti];
+    const int64_t ks = S > 1 ? split_slice_k(K, S) : K;
+    const int64_t slices = S > 1 ? cdiv(K, ks) : 1;
+    const int64_t passes = cdiv(cdiv(M, t.m) * cdiv(N, t.n) * slices, cu_count());
+    double c = (double)passes * (2.0 * t.m * t.n * (double)ks / (rate * eff[ti]) + 4.0);
+    if (slices > 1) c += ((double)slices * M * N * 4 + (double)M * N * osz * (use_c ? 2 : 1)) / 4e6 + 4.0;
+    return c;
+}
+
+struct SplitChoice {
+    Tile t;
+    int S;
+};
+// The split with the least modelled time against the no-split plan `base`
+// (the tile pick_tile or the caller chose); a split is taken only when the
+// model puts it at least 15 % ahead.  With `fixed` the tile stays the caller's.
+SplitChoice pick_split(int64_t M, int64_t N, int64_t K, int mode, int osz, bool use_c, const Tile* fixed, Tile base) {
+    int bi = 0;
+    for (int i = 0; i < 5; ++i)
+        if (kTiles[i].m == base.m && kTiles[i].n == base.n) bi = i;
+    const double none = plan_cost(M, N, K, mode, osz, use_c, bi, 1);
+    SplitChoice best{base, 1};
+    double best_c = none * 0.85;
+    static const int cand[] = {2, 3, 4, 6, 8, 12, 16};
+    for (int i = 0; i < 5; ++i) {
+        if (fixed && i != bi) continue;
+        for (int S : cand) {
+            if (K / S < split_min_k(mode)) break;
+            const double c = plan_cost(M, N, K, mode, osz, use_c, i, S);
+            if (c < best_c) { best_c = c; best = SplitChoice{kTiles[i], S}; }
+        }
+    }
+    return best;
+}
+
 int default_group_m(int64_t tiles_m, int64_t tiles_n) {
     int64_t per_xcd = std::max<int64_t>(1, tiles_m * tiles_n / 8);
     int g = 1;
@@ -145,6 +200,8 @@ struct Launch {
     int mfma;   // 0: 32 x 32 MFMAs; 16: the 16 x 16 x 32 form
     TileArgs args;
     int64_t grid;
+    int split;   // K slices (1: none)
+    int64_t ks;  // K per slice
 };
 
 bool aligned16(const void* p) { return ((uintptr_t)p & 15u) == 0; }
@@ -214,6 +271,10 @@ int read_tune_table(const std::string& path, std::map<TuneKey, ffgemm_tile>& out
             t.mfma_mn = (int)strtol(v[10].c_str(), &end, 10);
             if (end == v[10].c_str()) return -1;
         }
+        if (v.size() >= 13) {  // split_k before tflops
+            t.split_k = (int)strtol(v[11].c_str(), &end, 10);
+            if (end == v[11].c_str() || t.split_k < 0) return -1;
+        }
         out[TuneKey((int)x[0], layout, x[2], x[3], x[4])] = t;
         ++n;
     }
@@ -234,8 +295,10 @@ bool tuned_tile(const ffgemm_problem* p, ffgemm_tile* t) {
 }
 
 // Checks the problem against what the kernels support and fills `L`.
+// allow_split: the caller has a workspace for split-K partials (L.split may
+// then be > 1).
 int prepare_with(const ffgemm_problem* p, const void* A, const void* B, const void* C, void* D,
-                 const ffgemm_tile* tile, Launch& L) {
+                 const ffgemm_tile* tile, Launch& L, bool allow_split = false) {
     if (!p) return fail(FFGEMM_ERR, "null problem");
     if (p->layout < 0 || p->layout > 3) return fail(FFGEMM_ERR, "bad layout %d", p->layout);
     const int64_t M = p->M, N = p->N, K = p->K;
@@ -272,6 +335,7 @@ int prepare_with(const ffgemm_problem* p, const void* A, const void* B, const vo
 
     if (M == 0 || N == 0) {  // nothing to compute or store
         L.mode = mode; L.out = out; L.layout = p->layout; L.tile = kTiles[4]; L.grid = 0; L.waves = 0;
+        L.split = 1; L.ks = K;
         memset(&L.args, 0, sizeof(L.args));
         return FFGEMM_SUCCESS;
     }
@@ -340,6 +404,30 @@ int prepare_with(const ffgemm_problem* p, const void* A, const void* B, const vo
         t = Tile{tile->mt_m, tile->mt_n};
         group_m = tile->group_m;
     }
+    // split-K: only with a workspace for the fp32 partial planes, for single
+    // problems with an fp32 partial form (not bf16x3), each slice at least
+    // four K-tiles long
+    const bool split_ok = allow_split && batch == 1 && mode != kX3 && K >= 2 * split_min_k(mode);
+    int split = 1;
+    if (tile && tile->split_k < 0) return fail(FFGEMM_ERR, "split_k %d < 0", tile->split_k);
+    if (tile && tile->mt_m) {
+        // a caller's tile splits only when it says so
+        if (tile->split_k > 0) split = tile->split_k;
+    } else if (split_ok) {
+        const SplitChoice c = pick_split(M, N, K, mode, out == kOutF32 ? 4 : 2, use_c, nullptr, t);
+        if (c.S > 1) { t = c.t; split = c.S; }
+    }
+    int64_t ks = K;
+    if (split > 1) {
+        if (!split_ok)
+            return fail(FFGEMM_ERR, "split_k %d needs ffgemm_run_ws with a workspace, batch 1, not bf16x3, and K >= %lld",
+                        split, (long long)(2 * split_min_k(mode)));
+        ks = split_slice_k(K, split);
+        split = (int)cdiv(K, ks);
+        if (split < 2) ks = K;
+    }
+    L.split = split;
+    L.ks = ks;
     const int64_t tiles_m = cdiv(M, t.m), tiles_n = cdiv(N, t.n);
     const int64_t grid = tiles_m * tiles_n * batch;
     if (grid >= lim) return fail(FFGEMM_ERR, "too many tiles");
@@ -380,13 +468,15 @@ int64_t mn_k_chunk(const ffgemm_problem* p) {
 // tile for it (single problems in the default compute and output type); an
 // entry that does not fit the problem is passed over.
 int prepare(const ffgemm_problem* p, const void* A, const void* B, const void* C, void* D,
-            const ffgemm_tile* tile, Launch& L) {
+            const ffgemm_tile* tile, Launch& L, bool allow_split = false) {
     ffgemm_tile tuned;
     if (p && (!tile || !tile->mt_m) && p->batch <= 1 && p->compute == FFGEMM_COMPUTE_DEFAULT &&
-        (p->dtype_out == 0 || p->dtype_out == p->dtype || p->dtype == FFGEMM_F8E4M3) && tuned_tile(p, &tuned) &&
-        prepare_with(p, A, B, C, D, &tuned, L) == FFGEMM_SUCCESS)
-        return FFGEMM_SUCCESS;
-    return prepare_with(p, A, B, C, D, tile, L);
+        (p->dtype_out == 0 || p->dtype_out == p->dtype || p->dtype == FFGEMM_F8E4M3) && tuned_tile(p, &tuned)) {
+        // an entry with split-K runs as recorded with a workspace, else unsplit
+        if (!allow_split && tuned.split_k > 1) tuned.split_k = 1;
+        if (prepare_with(p, A, B, C, D, &tuned, L, allow_split) == FFGEMM_SUCCESS) return FFGEMM_SUCCESS;
+    }
+    return prepare_with(p, A, B, C, D, tile, L, allow_split);
 }
 
 int launch(const Launch& L, hipStream_t s) {
@@ -526,6 +616,7 @@ int ffgemm_query_tile(const ffgemm_problem* p, ffgemm_tile* out) {
         out->waves_m = geo.waves_m; out->waves_n = geo.waves_n; out->mfma_mn = L.mfma ? L.mfma : 32;
         out->group_m = L.args.group_m;
         out->lds_buffers = geo.lds_buffers;
+        out->split_k = 1;
     }
     return FFGEMM_SUCCESS;
 }
@@ -635,20 +726,124 @@ int64_t x3_ws_bytes(const ffgemm_problem* p) {
     const int64_t row = 3 * x3_kp(p->K) * 2;
     return round256(p->M * row * batch) + round256(p->N * row * batch);
 }
+
+// Split-K workspace: one fp32 plane of ldw x N per slice.
+int64_t split_ws_bytes(const Launch& L) { return round256((int64_t)L.split * split_ldw(L.args.M) * L.args.N * 4); }
+
+// The plan ffgemm_run_ws follows for a problem without bf16x3 planes: split-K
+// allowed (problems that run in K chunks do not split).
+int split_plan(const ffgemm_problem* p, const void* A, const void* B, const void* C, void* D,
+               const ffgemm_tile* tile, Launch& L) {
+    return prepare(p, A, B, C, D, tile, L, mn_k_chunk(p) == 0);
+}
+
+// Runs a split plan: slices of ks K as one batched GEMM of fp32 partials into
+// the workspace (alpha 1, beta 0; a shorter last slice as a second launch),
+// then splitk_reduce_kernel forms D = alpha * sum + beta * C.
+int run_split(const ffgemm_problem* p, const void* A, const void* B, const void* C, void* D, const Launch& L,
+              float* ws, hipStream_t s) {
+    const int S = L.split;
+    const int64_t ks = L.ks, M = p->M, N = p->N, K = p->K;
+    const bool ta = p->layout >= FFGEMM_TN, tb = (p->layout & 1) != 0;
+    const int es = mode_elem_bytes(L.mode);
+    const int64_t ldw = split_ldw(M);
+    ffgemm_tile t;
+    memset(&t, 0, sizeof(t));
+    t.mt_m = L.tile.m; t.mt_n = L.tile.n; t.mfma_mn = L.mfma; t.split_k = 1;
+    if (L.waves) { t.waves_m = 2; t.waves_n = L.waves / 2; }
+    ffgemm_problem q = *p;
+    q.dtype_out = FFGEMM_F32; q.round = FFGEMM_ROUND_RNE; q.alpha = 1.0f; q.beta = 0.0f;
+    q.ldc = q.ldd = ldw; q.bsC = q.bsD = ldw * N;
+    q.K = ks; q.bsA = ta ? ks : ks * p->lda; q.bsB = tb ? ks * p->ldb : ks;
+    const int64_t last = K - (int64_t)(S - 1) * ks;
+    const int nb = last == ks ? S : S - 1;
+    q.batch = nb;
+    int r = ffgemm_run(&q, A, B, nullptr, ws, &t, s);
+    if (r != FFGEMM_SUCCESS) return r;
+    if (nb < S) {
+        q.batch = 1;
+        q.K = last;
+        const int64_t k0 = (int64_t)(S - 1) * ks;
+        const void* Al = (const char*)A + (ta ? k0 : k0 * p->lda) * es;
+        const void* Bl = (const char*)B + (tb ? k0 * p->ldb : k0) * es;
+        r = ffgemm_run(&q, Al, Bl, nullptr, ws + (int64_t)(S - 1) * ldw * N, &t, s);
+        if (r != FFGEMM_SUCCESS) return r;
+    }
+    SplitReduceArgs ra;
+    ra.W = ws; ra.slice = ldw * N; ra.S = S; ra.M = (int32_t)M; ra.N = (int32_t)N; ra.ldw = (int32_t)ldw;
+    ra.C = p->beta != 0.0f ? C : nullptr; ra.D = D; ra.ldc = (int32_t)p->ldc; ra.ldd = (int32_t)p->ldd;
+    ra.alpha = p->alpha; ra.beta = p->beta; ra.out = L.out;
+    ra.trunc = (L.out == kOutBF16 && p->round == FFGEMM_ROUND_TRUNCATE) ? 1 : 0;
+    const int64_t work = cdiv(M, 4) * N;
+    const int blocks = (int)std::min<int64_t>(cdiv(work, 256), 8 * 65536);
+    hipLaunchKernelGGL(splitk_reduce_kernel, dim3(blocks), dim3(256), 0, s, ra);
+    hipError_t e = hipGetLastError();
+    return e == hipSuccess ? FFGEMM_SUCCESS : hip_fail(e, "split-K reduce launch");
+}
 }  // namespace
 
 extern "C" {
 
 int ffgemm_workspace_size(const ffgemm_problem* p, size_t* bytes) {
     if (!p || !bytes) return fail(FFGEMM_ERR, "null pointer");
-    *bytes = x3_planes_apply(p) ? (size_t)x3_ws_bytes(p) : 0;
+    *bytes = 0;
+    if (x3_planes_apply(p)) {
+        *bytes = (size_t)x3_ws_bytes(p);
+        return FFGEMM_SUCCESS;
+    }
+    // the split-K partial planes of the plan ffgemm_run_ws would follow
+    // (validation with dummy aligned pointers; nothing is dereferenced)
+    static const int64_t dummy[4] = {0, 0, 0, 0};
+    Launch L;
+    if (p->dtype != FFGEMM_F64 && split_plan(p, dummy, dummy, dummy, (void*)dummy, nullptr, L) == FFGEMM_SUCCESS &&
+        L.split > 1)
+        *bytes = (size_t)split_ws_bytes(L);
+    return FFGEMM_SUCCESS;
+}
+
+int ffgemm_query_tile_ws(const ffgemm_problem* p, ffgemm_tile* out) {
+    if (!p || !out) return fail(FFGEMM_ERR, "null pointer");
+    if (x3_planes_apply(p) || p->dtype == FFGEMM_F64) return ffgemm_query_tile(p, out);
+    static const int64_t dummy[4] = {0, 0, 0, 0};
+    Launch L;
+    const int r = split_plan(p, dummy, dummy, dummy, (void*)dummy, out->mt_m ? out : nullptr, L);
+    if (r != FFGEMM_SUCCESS) return r;
+    if (L.split <= 1) return ffgemm_query_tile(p, out);
+    const EngineGeometry geo = engine_geometry(L.mode, L.layout >= FFGEMM_TN, (L.layout & 1) == 0, L.tile.m,
+                                               L.tile.n, L.waves, L.mfma);
+    memset(out, 0, sizeof(*out));
+    out->mt_m = L.tile.m; out->mt_n = L.tile.n; out->depth_u = geo.depth_u;
+    out->waves_m = geo.waves_m; out->waves_n = geo.waves_n; out->mfma_mn = L.mfma ? L.mfma : 32;
+    out->group_m = L.args.group_m; out->lds_buffers = geo.lds_buffers; out->split_k = L.split;
     return FFGEMM_SUCCESS;
 }
 
 int ffgemm_run_ws(const ffgemm_problem* p, const void* A, const void* B, const void* C, void* D,
                   const ffgemm_tile* tile, void* workspace, size_t workspace_bytes, void* stream) {
     if (!p) return fail(FFGEMM_ERR, "null problem");
-    if (!x3_planes_apply(p) || !workspace || workspace_bytes < (size_t)x3_ws_bytes(p) || mn_k_chunk(p))
+    if (!x3_planes_apply(p)) {
+        if (p->dtype == FFGEMM_F64) return ffgemm_run(p, A, B, C, D, tile, stream);
+        Launch L;
+        const int r = split_plan(p, A, B, C, D, tile, L);
+        if (r != FFGEMM_SUCCESS) return r;
+        if (L.split > 1) {
+            const int64_t need = split_ws_bytes(L);
+            if (workspace && workspace_bytes >= (size_t)need) {
+                if ((uintptr_t)workspace % 256) return fail(FFGEMM_ERR, "workspace must be 256-byte aligned");
+                return run_split(p, A, B, C, D, L, (float*)workspace, (hipStream_t)stream);
+            }
+            if (tile && tile->mt_m && tile->split_k > 1)
+                return fail(FFGEMM_ERR, "split_k %d needs a workspace of %lld bytes (%zu given)", tile->split_k,
+                            (long long)need, workspace ? workspace_bytes : (size_t)0);
+        }
+        if (tile && tile->split_k > 1) {  // a requested split that came to one slice
+            ffgemm_tile t = *tile;
+            t.split_k = 1;
+            return ffgemm_run(p, A, B, C, D, &t, stream);
+        }
+        return ffgemm_run(p, A, B, C, D, tile, stream);
+    }
+    if (!workspace || workspace_bytes < (size_t)x3_ws_bytes(p) || mn_k_chunk(p))
         return ffgemm_run(p, A, B, C, D, tile, stream);
     if ((uintptr_t)workspace % 256) return fail(FFGEMM_ERR, "workspace must be 256-byte aligned");
     Launch L;

@@ -88,6 +88,23 @@ struct HalfLoader {
             __builtin_amdgcn_raw_ptr_buffer_load_lds(rsrc, (lds_ptr_t)(dst + q * 1024), 16, off, 0, 0, 0);
         }
     }
+    // Per-q lane offsets for issue_full: rows past the matrix at kOOB.
+    __device__ inline void full_offsets(uint32_t (&v)[NLD]) const {
+#pragma unroll
+        for (int q = 0; q < NLD; ++q)
+            v[q] = (QROWS * q < rlim) ? base + q * qstride + (uint32_t)(x ^ q) * 16 : kOOB;
+    }
+    // issue() for a K-tile that lies wholly inside K (kt * DU + DU <= K): the
+    // K-tile's byte offset goes in the instruction's scalar offset, so the loads
+    // need no per-lane arithmetic.  Rows past the matrix read zero through
+    // their kOOB lane offset (out of range with or without the scalar part).
+    __device__ inline void issue_full(int kt, char* stage, const uint32_t (&v)[NLD]) const {
+        char* dst = stage + lds_off;
+        const int soff = __builtin_amdgcn_readfirstlane(kt * RB);
+#pragma unroll
+        for (int q = 0; q < NLD; ++q)
+            __builtin_amdgcn_raw_ptr_buffer_load_lds(rsrc, (lds_ptr_t)(dst + q * 1024), 16, v[q], soff, 0, 0);
+    }
 };
 
 // Half (128 rows) of a 256-row M/N-contiguous 16-bit operand for one K-tile,
@@ -127,6 +144,10 @@ struct MNHalfLoader {
             __builtin_amdgcn_raw_ptr_buffer_load_lds(rsrc, (lds_ptr_t)(dst + q * 1024), 16, off, 0, 0, 0);
         }
     }
+    // (no scalar-offset form: rows past the matrix are masked only by the
+    // range check of the per-lane offset)
+    __device__ inline void full_offsets(uint32_t (&)[NLD]) const {}
+    __device__ inline void issue_full(int kt, char* region, const uint32_t (&)[NLD]) const { issue(kt, region); }
 };
 
 typedef short i16x4_t __attribute__((ext_vector_type(4)));
@@ -245,9 +266,26 @@ __device__ inline void gemm_tile(const TileArgs& a, int tm, int tn, int b, char*
     lds_barrier();
     if (grp == 1) barrier();  // group 1 runs one window behind group 0
 
+    // K-contiguous operands: K-tiles wholly inside K load with fixed per-lane
+    // offsets and the K position in the instruction's scalar offset (no
+    // per-lane arithmetic in the windows); the tail and the K-tiles past the
+    // end take the masked path, which reads zeros beyond K
+    uint32_t va[NLD], vb[NLD];
+    if constexpr (AK) la.full_offsets(va);
+    if constexpr (BK) lb.full_offsets(vb);
+    const int KF = a.K / DU;
+    auto loadB = [&](int kt, auto full) {
+        if constexpr (BK && decltype(full)::value) lb.issue_full(kt, stB(kt) + lhalf_b, vb);
+        else lb.issue(kt, stB(kt) + lhalf_b);
+    };
+    auto loadA = [&](int kt, char* stage, auto full) {
+        if constexpr (AK && decltype(full)::value) la.issue_full(kt, stage + lhalf_a, va);
+        else la.issue(kt, stage + lhalf_a);
+    };
+
     // Windows per K-tile: memory(half 0), compute(half 0), memory(half 1),
     // compute(half 1); a half is K = 32 (two MFMA K-steps).
-    for (int kt = 0; kt < KT; ++kt) {
+    auto ktile = [&](int kt, auto full) {
         const char* sa = stA(kt) + aoff;
         const char* sb = stB(kt) + boff;
         const char* ra = stA(kt) + aregion;
@@ -255,7 +293,7 @@ __device__ inline void gemm_tile(const TileArgs& a, int tm, int tn, int b, char*
 #pragma unroll
         for (int half = 0; half < 2; ++half) {
             // ---- memory window
-            if (half == 0) lb.issue(kt + 2, stB(kt + 2) + lhalf_b);
+            if (half == 0) loadB(kt + 2, full);
 #pragma unroll
             for (int k2 = 0; k2 < 2; ++k2) {
                 const int ks = 2 * half + k2;
@@ -274,7 +312,7 @@ __device__ inline void gemm_tile(const TileArgs& a, int tm, int tn, int b, char*
             lds_barrier();
             __builtin_amdgcn_sched_barrier(0);
             // ---- compute window
-            if (half == 1) la.issue(kt + 2, stA(kt) + lhalf_a);  // this group's half of A is consumed
+            if (half == 1) loadA(kt + 2, stA(kt), full);  // this group's half of A is consumed
             __builtin_amdgcn_s_setprio(1);
             if constexpr (MODE == kF8) {
 #pragma unroll
@@ -294,7 +332,11 @@ __device__ inline void gemm_tile(const TileArgs& a, int tm, int tn, int b, char*
             if (half == 1) wait_vmcnt<2 * NLD>();
             barrier();
         }
-    }
+    };
+    // K-tiles whose loads (of K-tile kt+2) lie wholly inside K, then the rest
+    int kt = 0;
+    for (; kt < KF - 2; ++kt) ktile(kt, std::true_type{});
+    for (; kt < KT; ++kt) ktile(kt, std::false_type{});
     // loads issued past the end write zeros into free stages; drain them
     // before the workgroup's LDS is released
     wait_vmcnt<0>();

@@ -60,13 +60,14 @@ def make_problem(layout, dtype, M, N, K, lda, ldb, ldc, ldd, alpha=1.0, beta=0.0
 
 
 def make_tile(tile) -> C.Tile | None:
-    """Tile record from (mt_m, mt_n[, group_m[, waves_m, waves_n[, mfma_mn]]])
+    """Tile record from (mt_m, mt_n[, group_m[, waves_m, waves_n[, mfma_mn[, split_k]]]])
     or a Tile.
 
     waves_m x waves_n picks the wave tiling: 2 x 4 the eight-wave engine,
     2 x 2 a four-wave one, 0 x 0 (or absent) the tile's default.  mfma_mn 16
     picks 16 x 16 x 32 MFMAs (bf16 / fp16 at 64 x 64 and 128 x 128); 0 or 32
-    the 32 x 32 ones."""
+    the 32 x 32 ones.  split_k >= 2 cuts K into that many slices (with a
+    workspace only)."""
     if tile is None:
         return None
     if isinstance(tile, C.Tile):
@@ -79,6 +80,8 @@ def make_tile(tile) -> C.Tile | None:
         t.waves_m, t.waves_n = int(tile[3]), int(tile[4])
     if len(tile) > 5:
         t.mfma_mn = int(tile[5])
+    if len(tile) > 6:
+        t.split_k = int(tile[6])
     return t
 
 
@@ -121,6 +124,13 @@ def run(problem: C.Problem, A, B, Cm, D, tile=None, stream=None, workspace="auto
 def query_tile(problem: C.Problem, tile=None) -> C.Tile:
     t = make_tile(tile) or C.Tile()
     C.check(C.lib().ffgemm_query_tile(ctypes.byref(problem), ctypes.byref(t)))
+    return t
+
+
+def query_tile_ws(problem: C.Problem, tile=None) -> C.Tile:
+    """The record ffgemm_run_ws follows with a workspace (split_k included)."""
+    t = make_tile(tile) or C.Tile()
+    C.check(C.lib().ffgemm_query_tile_ws(ctypes.byref(problem), ctypes.byref(t)))
     return t
 
 

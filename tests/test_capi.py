@@ -267,11 +267,11 @@ def test_tuned_tile_lookup(tmp_path, monkeypatch):
         autotune.save_db({autotune.key(C.BF16, "TN", 4096, 4096, 4096): {"tile": [256, 256, 0, 2, 2], "tflops": 1.0},
                           autotune.key(C.F16, "NN", 480, 512, 512): {"tile": [64, 64, 0, 0, 0, 16], "tflops": 1.0}})
         p = _p("TN", C.BF16, 4096, 4096, 4096)
-        assert list(G._tuned(p)) == [256, 256, 0, 2, 2, 0]
+        assert list(G._tuned(p)) == [256, 256, 0, 2, 2, 0, 0]
         t = tl.query_tile(p, G.make_tile(G._tuned(p)))
         assert (t.waves_m, t.waves_n, t.mfma_mn) == (2, 2, 32)
         p16 = _p("NN", C.F16, 480, 512, 512)
-        assert list(G._tuned(p16)) == [64, 64, 0, 0, 0, 16]
+        assert list(G._tuned(p16)) == [64, 64, 0, 0, 0, 16, 0]
         assert tl.query_tile(p16).mfma_mn == 16        # the C library reads the same table
         assert G._tuned(_p("TN", C.BF16, 1024, 1024, 1024)) is None
     finally:
@@ -328,9 +328,12 @@ def test_committed_tune_table_is_valid():
             dt, lay, mnk = k.split(":")
             M, N, K = (int(x) for x in mnk.split("x"))
             p = _p(lay, int(dt), M, N, K)
-            t = tl.query_tile(p)
-            mt_m, mt_n, gm, wm, wn, mf = e["tile"]
+            mt_m, mt_n, gm, wm, wn, mf, sk = e["tile"]
+            # an entry with split-K is followed on the workspace path
+            t = tl.query_tile_ws(p) if sk > 1 else tl.query_tile(p)
             assert (t.mt_m, t.mt_n, t.mfma_mn) == (mt_m, mt_n, mf or 32), k
+            if sk > 1:
+                assert t.split_k == sk, k
             if wm:
                 assert (t.waves_m, t.waves_n) == (wm, wn), k
             if gm:
@@ -428,3 +431,46 @@ def test_grouped_plan_create_rejects_non_16bit_groups(kind):
     assert plan.value is None
     msg = C.lib().ffgemm_last_error().decode()
     assert ("bf16 or fp16" in msg) or ("dtype" in msg), msg
+
+
+def test_split_k_planning():
+    """Split-K is planned only for the workspace path: ffgemm_query_tile (the
+    ffgemm_run plan) never splits, ffgemm_query_tile_ws does for a problem with
+    few output tiles, and the workspace it needs is S fp32 planes of
+    roundup(M, 4) x N."""
+    p = _p("TN", C.F32, 480, 2048, 2048)
+    assert tl.query_tile(p).split_k == 1
+    t = tl.query_tile_ws(p)
+    assert t.split_k >= 2
+    ldw = (480 + 3) // 4 * 4
+    assert tl.workspace_size(p) == (t.split_k * ldw * 2048 * 4 + 255) // 256 * 256
+    # plenty of tiles: no split, no workspace
+    q = _p("TN", C.BF16, 4096, 4096, 4096)
+    assert tl.query_tile_ws(q).split_k == 1 and tl.workspace_size(q) == 0
+    # a caller's record with split_k: followed on the workspace path
+    t = tl.query_tile_ws(q, (128, 128, 0, 0, 0, 0, 3))
+    assert (t.mt_m, t.mt_n, t.split_k) == (128, 128, 3)
+
+
+@pytest.mark.parametrize("case", ["run", "batch", "x3", "short_k", "negative"])
+def test_split_k_rejections(case):
+    """A split the problem cannot take is refused with RTN_ERR: through
+    ffgemm_run (no workspace), with batch > 1, for bf16x3, with K below two
+    minimal slices, or a negative count."""
+    fake = ctypes.c_void_p(0x100000)
+    t = C.Tile()
+    t.mt_m, t.mt_n, t.split_k = 128, 128, (-1 if case == "negative" else 4)
+    if case == "batch":
+        p = tl.make_problem("TN", C.BF16, 512, 512, 4096, 4096, 4096, 512, 512, batch=2, bsA=512 * 4096,
+                            bsB=512 * 4096, bsD=512 * 512)
+    elif case == "x3":
+        p = _p("NN", C.F32, 512, 512, 4096, compute=C.COMPUTE_BF16X3)
+    elif case == "short_k":
+        p = _p("TN", C.BF16, 512, 512, 256)
+    else:
+        p = _p("TN", C.BF16, 512, 512, 4096)
+    with pytest.raises(C.FFGemmError, match="RTN_ERR"):
+        tl.query_tile_ws(p, t) if case != "run" else tl.query_tile(p, t)
+    if case == "run":
+        st = C.lib().ffgemm_run(ctypes.byref(p), fake, fake, None, fake, ctypes.byref(t), None)
+        assert st == C.ERR and "workspace" in C.lib().ffgemm_last_error().decode()
