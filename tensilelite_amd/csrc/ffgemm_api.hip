@@ -129,7 +129,12 @@ int64_t split_ldw(int64_t M) { return cdiv(M, 4) * 4; }
 // 4 µs of prologue and epilogue; for S > 1 also the reduction (S fp32 planes
 // read, D written, C read, at 4 TB/s) and 4 µs for its launch.
 double plan_cost(int64_t M, int64_t N, int64_t K, int mode, int osz, bool use_c, int ti, int S) {
-    static const double eff[] = {1.0, 0.82, 0.82, 0.70, 0.51};  // kTiles order
+    // relative efficiency per tile (kTiles order): 16-bit / fp8 from the
+    // 4096^3 sweep; fp32 nearly flat (the phantom-shape sweep on MI355X put
+    // its 64 x 64 and 128 x 128 tiles level with or ahead of 256 x 256)
+    static const double eff16[] = {1.0, 0.82, 0.82, 0.70, 0.51};
+    static const double eff32[] = {0.85, 0.9, 0.9, 0.95, 1.0};
+    const double* eff = mode == kF32 ? eff32 : eff16;
     const double rate = (mode == kF8 ? 8192.0 : mode == kF32 ? 256.0 : 4096.0) * 1.9e3 * 0.7;  // flop/µs/CU
     const Tile& t = kTiles[ti];
     const int64_t ks = S > 1 ? split_slice_k(K, S) : K;

@@ -438,12 +438,12 @@ def test_split_k_planning():
     ffgemm_run plan) never splits, ffgemm_query_tile_ws does for a problem with
     few output tiles, and the workspace it needs is S fp32 planes of
     roundup(M, 4) x N."""
-    p = _p("TN", C.F32, 480, 2048, 2048)
+    p = _p("TN", C.F32, 480, 1024, 1024)
     assert tl.query_tile(p).split_k == 1
     t = tl.query_tile_ws(p)
     assert t.split_k >= 2
     ldw = (480 + 3) // 4 * 4
-    assert tl.workspace_size(p) == (t.split_k * ldw * 2048 * 4 + 255) // 256 * 256
+    assert tl.workspace_size(p) == (t.split_k * ldw * 1024 * 4 + 255) // 256 * 256
     # plenty of tiles: no split, no workspace
     q = _p("TN", C.BF16, 4096, 4096, 4096)
     assert tl.query_tile_ws(q).split_k == 1 and tl.workspace_size(q) == 0
