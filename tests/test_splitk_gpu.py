@@ -83,8 +83,9 @@ def test_split_k_fp8(cuda):
     B = rng.standard_normal(N * K).astype(np.float32)
     sa, sb = np.float32(oracle.fp8_scale(np.abs(A).max())), np.float32(oracle.fp8_scale(np.abs(B).max()))
     qa, qb = oracle.f32_to_e4m3_bits(A, sa), oracle.f32_to_e4m3_bits(B, sb)
-    p = tl.make_problem("TN", C.F8E4M3, M, N, K, K, K, M, M, scaleA=torch.tensor([sa], device=cuda),
-                        scaleB=torch.tensor([sb], device=cuda), dtype_out=C.F32)
+    # the problem holds the scales' device pointers: keep the tensors alive
+    dsa, dsb = torch.tensor([sa], device=cuda), torch.tensor([sb], device=cuda)
+    p = tl.make_problem("TN", C.F8E4M3, M, N, K, K, K, M, M, scaleA=dsa, scaleB=dsb, dtype_out=C.F32)
     rec = (128, 128, 0, 0, 0, 0, 3)
     ws = torch.empty(3 * M * N * 4 + 256, dtype=torch.uint8, device=cuda)
     D = torch.empty(M * N, device=cuda)
