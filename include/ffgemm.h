@@ -147,7 +147,15 @@ int ffgemm_tune_db_load(const char *path, int *entries);
 /* D = alpha*op(A)*op(B) + beta*C on `stream` (hipStream_t, NULL = null stream).
  * Replaces the path GemmMfmaAsmSolution::generateKernel -> StreamHip::Launch
  * (src/gemmMfma.cpp:24-185, ff_gpu_src/ff_hip_runtime.cpp:278-320).
- * `tile` may be NULL.  beta == 0 never reads C. */
+ * `tile` may be NULL.  beta == 0 never reads C.
+ * Leading dimensions and batch strides of any length, any K, and pointers
+ * aligned to their element size are accepted.  Problems with every ld, batch
+ * stride and pointer on the 16-byte grid (and K a multiple of 16 bytes of
+ * elements for a K-contiguous operand) run the vector engines; any other runs
+ * the general-stride kernel (element-wise loads and stores, MT64x64 or
+ * MT128x128 with 2x2 waves and 2 LDS stages; fp8 MT128x128), which a tile
+ * record can ask for and which never splits K.  Grouped plans take grid-
+ * aligned problems only. */
 int ffgemm_run(const ffgemm_problem *p, const void *A, const void *B, const void *C, void *D,
                const ffgemm_tile *tile, void *stream);
 

@@ -35,7 +35,7 @@ DEVICE_FLAGS = ["-O3", "-std=c++17", "-fPIC", f"--offload-arch={ARCH}",
                 "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
                 "-Wno-unused-command-line-argument"]
 
-KERNEL_SOURCES = ["gemm_bf16.hip", "gemm_f16.hip", "gemm_f32.hip", "gemm_x3.hip", "gemm_f8.hip",
+KERNEL_SOURCES = ["gemm_bf16.hip", "gemm_f16.hip", "gemm_f32.hip", "gemm_x3.hip", "gemm_f8.hip", "gemm_any16.hip", "gemm_any32.hip",
                   "grouped.hip", "aux_kernels.hip", "ffgemm_api.hip"]
 HARNESS_SOURCES = ["harness/ff_runtime.cpp", "harness/gemm_problem.cpp", "harness/main.cpp"]
 

@@ -207,6 +207,7 @@ struct Launch {
     int64_t grid;
     int split;   // K slices (1: none)
     int64_t ks;  // K per slice
+    bool any;    // the general-stride form (gemm_kernel_any)
 };
 
 bool aligned16(const void* p) { return ((uintptr_t)p & 15u) == 0; }
@@ -340,27 +341,27 @@ int prepare_with(const ffgemm_problem* p, const void* A, const void* B, const vo
 
     if (M == 0 || N == 0) {  // nothing to compute or store
         L.mode = mode; L.out = out; L.layout = p->layout; L.tile = kTiles[4]; L.grid = 0; L.waves = 0;
-        L.split = 1; L.ks = K;
+        L.split = 1; L.ks = K; L.any = false;
         memset(&L.args, 0, sizeof(L.args));
         return FFGEMM_SUCCESS;
     }
     const int es = mode_elem_bytes(mode);
     const int ve = 16 / es;          // elements per 16-byte vector
     const int os = out == kOutF32 ? 4 : 2;
-    if ((p->lda * es) % 16 || (p->ldb * es) % 16)
-        return fail(FFGEMM_ERR, "lda/ldb must be multiples of %d elements (16-byte rows)", ve);
-    if ((p->ldd * os) % 16 || (use_c && (p->ldc * os) % 16))
-        return fail(FFGEMM_ERR, "ldc/ldd must be multiples of %d elements", 16 / os);
-    if ((ta || !tb) && K % ve)
-        return fail(FFGEMM_ERR, "K must be a multiple of %d for a K-contiguous operand", ve);
     if (M && N && (!A || !B || !D)) return fail(FFGEMM_ERR, "null matrix pointer");
     if (use_c && M && N && !C) return fail(FFGEMM_ERR, "beta != 0 but C is null");
-    if (!aligned16(A) || !aligned16(B) || !aligned16(D) || (use_c && !aligned16(C)))
-        return fail(FFGEMM_ERR, "matrix pointers must be 16-byte aligned");
-    if (batch > 1) {
-        if ((p->bsA * es) % 16 || (p->bsB * es) % 16 || (p->bsD * os) % 16 || (use_c && (p->bsC * os) % 16))
-            return fail(FFGEMM_ERR, "batch strides must keep 16-byte alignment");
-    }
+    // The engines move 16-byte vectors: leading dimensions, batch strides and
+    // pointers at 16-byte multiples, K a multiple of the vector for a
+    // K-contiguous operand.  Any other problem runs the general-stride form
+    // (element-wise loads and stores), which needs only element alignment.
+    const bool vec_ok = (p->lda * es) % 16 == 0 && (p->ldb * es) % 16 == 0 && (p->ldd * os) % 16 == 0 &&
+                        (!use_c || (p->ldc * os) % 16 == 0) && (!(ta || !tb) || K % ve == 0) &&
+                        aligned16(A) && aligned16(B) && aligned16(D) && (!use_c || aligned16(C)) &&
+                        (batch <= 1 || ((p->bsA * es) % 16 == 0 && (p->bsB * es) % 16 == 0 &&
+                                        (p->bsD * os) % 16 == 0 && (!use_c || (p->bsC * os) % 16 == 0)));
+    const bool any = !vec_ok;
+    if (any && (((uintptr_t)A % es) || ((uintptr_t)B % es) || ((uintptr_t)D % os) || (use_c && ((uintptr_t)C % os))))
+        return fail(FFGEMM_ERR, "matrix pointers must be aligned to their element size");
     // Buffer offsets are 31-bit.  A K-contiguous operand is addressed from each
     // tile's first row (at most 256 rows, and a margin row), so any number of
     // rows fits; an M/N-contiguous operand is addressed whole, all K rows of it.
@@ -373,10 +374,26 @@ int prepare_with(const ffgemm_problem* p, const void* A, const void* B, const vo
     // the in-register bf16x3 split does not fit 256 x 256 without spilling
     // (8192^3 on MI355X: 165-203 TFLOP/s there, 284-289 at 128 x 256)
     if (mode == kX3 && t.m == 256 && t.n == 256) t = Tile{128, 256};
+    // the general-stride form: 128 x 128 when that still gives a tile per CU
+    // (fp8 always: its 64 x 64 form spills)
+    if (any) t = (mode == kF8 || cdiv(M, 128) * cdiv(N, 128) * batch >= cu_count()) ? Tile{128, 128} : Tile{64, 64};
     int group_m = 0;
     int waves = 0;
     int mfma = 0;
-    if (tile && tile->mt_m) {
+    if (tile && tile->mt_m && any) {
+        // one engine, two square tiles, 32 x 32 MFMAs, 2 x 2 waves, two stages
+        const int du = mode_depth_u(mode);
+        if (!(((tile->mt_m == 64 && mode != kF8) || tile->mt_m == 128) && tile->mt_n == tile->mt_m) ||
+            (tile->waves_m && (tile->waves_m != 2 || tile->waves_n != 2)) ||
+            (tile->mfma_mn && tile->mfma_mn != 32) || (tile->lds_buffers && tile->lds_buffers != 2) ||
+            (tile->depth_u && tile->depth_u != du))
+            return fail(FFGEMM_ERR,
+                        "leading dimensions, batch strides, K or pointers off the 16-byte grid run the "
+                        "general-stride kernel: MT64x64 (not fp8) or MT128x128, 2x2 waves, 32x32 MFMAs, 2 LDS stages, "
+                        "depth_u %d (MT%dx%d asked)", du, tile->mt_m, tile->mt_n);
+        t = Tile{tile->mt_m, tile->mt_m};
+        group_m = tile->group_m;
+    } else if (tile && tile->mt_m) {
         if (!tile_supported(tile->mt_m, tile->mt_n))
             return fail(FFGEMM_ERR, "unsupported macro tile %dx%d", tile->mt_m, tile->mt_n);
         // waves_m x waves_n picks the wave tiling (0: the tile's default engine;
@@ -412,7 +429,7 @@ int prepare_with(const ffgemm_problem* p, const void* A, const void* B, const vo
     // split-K: only with a workspace for the fp32 partial planes, for single
     // problems with an fp32 partial form (not bf16x3), each slice at least
     // four K-tiles long
-    const bool split_ok = allow_split && batch == 1 && mode != kX3 && K >= 2 * split_min_k(mode);
+    const bool split_ok = allow_split && batch == 1 && mode != kX3 && !any && K >= 2 * split_min_k(mode);
     int split = 1;
     if (tile && tile->split_k < 0) return fail(FFGEMM_ERR, "split_k %d < 0", tile->split_k);
     if (tile && tile->mt_m) {
@@ -425,8 +442,8 @@ int prepare_with(const ffgemm_problem* p, const void* A, const void* B, const vo
     int64_t ks = K;
     if (split > 1) {
         if (!split_ok)
-            return fail(FFGEMM_ERR, "split_k %d needs ffgemm_run_ws with a workspace, batch 1, not bf16x3, and K >= %lld",
-                        split, (long long)(2 * split_min_k(mode)));
+            return fail(FFGEMM_ERR, "split_k %d needs ffgemm_run_ws with a workspace, batch 1, not bf16x3, "
+                        "16-byte-aligned strides and pointers, and K >= %lld", split, (long long)(2 * split_min_k(mode)));
         ks = split_slice_k(K, split);
         split = (int)cdiv(K, ks);
         if (split < 2) ks = K;
@@ -449,6 +466,7 @@ int prepare_with(const ffgemm_problem* p, const void* A, const void* B, const vo
     a.group_m = group_m > 0 ? group_m : default_group_m(tiles_m, tiles_n);
     a.flags = (use_c ? kFlagBeta : 0u) | (p->round == FFGEMM_ROUND_TRUNCATE ? kFlagTrunc : 0u);
     L.mode = mode; L.out = out; L.layout = p->layout; L.tile = t; L.grid = grid; L.waves = waves; L.mfma = mfma;
+    L.any = any;
     return FFGEMM_SUCCESS;
 }
 
@@ -484,9 +502,18 @@ int prepare(const ffgemm_problem* p, const void* A, const void* B, const void* C
     return prepare_with(p, A, B, C, D, tile, L, allow_split);
 }
 
+// The engine geometry of a plan, as ffgemm_query_tile reports it.
+EngineGeometry launch_geometry(const Launch& L) {
+    if (L.any) return {2, 2, mode_depth_u(L.mode), 2};
+    return engine_geometry(L.mode, L.layout >= FFGEMM_TN, (L.layout & 1) == 0, L.tile.m, L.tile.n, L.waves, L.mfma);
+}
+
 int launch(const Launch& L, hipStream_t s) {
     int r = -1;
-    switch (L.mode) {
+    if (L.any) {
+        r = (L.mode == kBF16 || L.mode == kF16) ? launch_any16(L.mode, L.layout, L.out, L.tile.m, L.args, (int)L.grid, s)
+                                                : launch_any32(L.mode, L.layout, L.out, L.tile.m, L.args, (int)L.grid, s);
+    } else switch (L.mode) {
         case kBF16: r = launch_bf16(L.layout, L.out, L.tile.m, L.tile.n, L.waves, L.mfma, L.args, (int)L.grid, s); break;
         case kF16: r = launch_f16(L.layout, L.out, L.tile.m, L.tile.n, L.waves, L.mfma, L.args, (int)L.grid, s); break;
         case kF32: r = launch_f32(L.layout, L.out, L.tile.m, L.tile.n, L.waves, L.mfma, L.args, (int)L.grid, s); break;
@@ -615,8 +642,7 @@ int ffgemm_query_tile(const ffgemm_problem* p, ffgemm_tile* out) {
     if (r != FFGEMM_SUCCESS) return r;
     if (out) {
         memset(out, 0, sizeof(*out));
-        const EngineGeometry geo = engine_geometry(L.mode, L.layout >= FFGEMM_TN, (L.layout & 1) == 0, L.tile.m,
-                                                   L.tile.n, L.waves, L.mfma);
+        const EngineGeometry geo = launch_geometry(L);
         out->mt_m = L.tile.m; out->mt_n = L.tile.n; out->depth_u = geo.depth_u;
         out->waves_m = geo.waves_m; out->waves_n = geo.waves_n; out->mfma_mn = L.mfma ? L.mfma : 32;
         out->group_m = L.args.group_m;
@@ -664,8 +690,14 @@ int ffgemm_tile_from_reference(const ffgemm_problem* p, int r, int s, int x, int
         memset(&t, 0, sizeof(t));
         t.mt_m = kTiles[best].m; t.mt_n = kTiles[best].n;
         ffgemm_tile q = t;
-        const int rc = ffgemm_query_tile(p, &q);
-        if (rc != FFGEMM_SUCCESS) return rc;
+        int rc = ffgemm_query_tile(p, &q);
+        if (rc != FFGEMM_SUCCESS) {
+            // no kernel has that tile for this problem (strides off the 16-byte
+            // grid run two square tiles only): the library's own choice
+            memset(&q, 0, sizeof(q));
+            rc = ffgemm_query_tile(p, &q);
+            if (rc != FFGEMM_SUCCESS) return rc;
+        }
         t = q;
     }
     *out = t;
@@ -814,8 +846,7 @@ int ffgemm_query_tile_ws(const ffgemm_problem* p, ffgemm_tile* out) {
     const int r = split_plan(p, dummy, dummy, dummy, (void*)dummy, out->mt_m ? out : nullptr, L);
     if (r != FFGEMM_SUCCESS) return r;
     if (L.split <= 1) return ffgemm_query_tile(p, out);
-    const EngineGeometry geo = engine_geometry(L.mode, L.layout >= FFGEMM_TN, (L.layout & 1) == 0, L.tile.m,
-                                               L.tile.n, L.waves, L.mfma);
+    const EngineGeometry geo = launch_geometry(L);
     memset(out, 0, sizeof(*out));
     out->mt_m = L.tile.m; out->mt_n = L.tile.n; out->depth_u = geo.depth_u;
     out->waves_m = geo.waves_m; out->waves_n = geo.waves_n; out->mfma_mn = L.mfma ? L.mfma : 32;
@@ -1055,6 +1086,9 @@ int ffgemm_grouped_plan_create(ffgemm_grouped_plan* plan, int count, const ffgem
         Launch L;
         int r = prepare(&p, A[i], B[i], C ? C[i] : nullptr, D[i], &tt, L);
         if (r != FFGEMM_SUCCESS) return r;
+        if (L.any)
+            return fail(FFGEMM_ERR, "grouped problems need leading dimensions and pointers on the 16-byte grid and K "
+                        "a multiple of 8 for a K-contiguous operand (problem %d)", i);
         if (i == 0) { mode = L.mode; out = L.out; layout = L.layout; }
         else if (L.mode != mode || L.out != out || L.layout != layout)
             return fail(FFGEMM_ERR, "grouped problems must share dtype, compute and layout");

@@ -126,6 +126,39 @@ inline int launch_mode(int layout, int mt_m, int mt_n, int waves, int mfma, cons
     return -1;
 }
 
+// The general-stride form (gemm_kernel_any: element-wise loads and stores) at
+// MT x MT, MT = 64 or 128 (fp8: 128).  fp8 operands are K-contiguous (TN) only.
+template <int MODE, int OUT>
+inline int launch_any_mode(int layout, int mt, const TileArgs& a, int grid, hipStream_t s) {
+    auto go = [&](auto ak, auto bk) -> int {
+        constexpr bool AK = decltype(ak)::value, BK = decltype(bk)::value;
+        if constexpr (MODE == kF8 && !(AK && BK)) {
+            return -1;
+        } else {
+            if (mt == 64 && MODE != kF8)  // fp8 at 64 x 64 spills: 128 x 128 only
+                hipLaunchKernelGGL((gemm_kernel_any<MODE, AK, BK, (MODE == kF8 ? 128 : 64), OUT>), dim3(grid),
+                                   dim3(kThreads), 0, s, a);
+            else if (mt == 128)
+                hipLaunchKernelGGL((gemm_kernel_any<MODE, AK, BK, 128, OUT>), dim3(grid), dim3(kThreads), 0, s, a);
+            else
+                return -1;
+            return 0;
+        }
+    };
+    using F = std::false_type;
+    using T = std::true_type;
+    switch (layout) {
+        case 0: return go(F{}, T{});
+        case 1: return go(F{}, F{});
+        case 2: return go(T{}, T{});
+        case 3: return go(T{}, F{});
+    }
+    return -1;
+}
+// bf16 / fp16 (gemm_any16.hip); fp32, bf16x3 and fp8 (gemm_any32.hip)
+int launch_any16(int mode, int layout, int out, int mt, const TileArgs& a, int grid, hipStream_t s);
+int launch_any32(int mode, int layout, int out, int mt, const TileArgs& a, int grid, hipStream_t s);
+
 // Per-mode entry points (defined in gemm_<mode>.hip).
 // waves: 0 (default engine), 4 or 8; mfma: 0 / 32, or 16 (see engine_for)
 int launch_bf16(int layout, int out, int mt_m, int mt_n, int waves, int mfma, const TileArgs& a, int grid, hipStream_t s);

@@ -20,7 +20,11 @@ namespace ffgemm {
 // of an 8 (16-bit) or 4 (32-bit) element column strip and transposes it in
 // registers.
 // PF: register slots, i.e. K-tiles whose loads can be in flight at once.
-template <int MODE, bool KC, int MT, bool PERM, int PF = 1>
+// ANY: the general-stride form.  Each 16-byte register vector is gathered by
+// element loads (1, 2 or 4 bytes), so leading dimensions, batch strides and K
+// need no 16-byte multiple and pointers only their element alignment; elements
+// past K (and, M/N-contiguous, past the last row) read as zero.
+template <int MODE, bool KC, int MT, bool PERM, int PF = 1, bool ANY = false>
 struct Loader {
     static constexpr int ES = mode_elem_bytes(MODE);
     static constexpr int DU = mode_depth_u(MODE);
@@ -42,6 +46,7 @@ struct Loader {
     int32_t kidx[NLD];      // K index of each load for K-tile 0
     uint32_t kstep;         // bytes to advance per K-tile
     int32_t K;
+    int32_t rlim;           // M/N-contiguous, ANY: rows of the vector inside the matrix
     int tid;                // thread index in the workgroup
     i32x4_t v[PF][NLD];
 
@@ -65,6 +70,7 @@ struct Loader {
             constexpr int NC = MT / VE;  // column strips across the tile
             const int mc = t % NC, kg = t / NC;
             const int r = row0 + mc * VE;
+            rlim = rows - r;
 #pragma unroll
             for (int j = 0; j < KB; ++j) {
                 int k = kg * KB + j;
@@ -81,10 +87,49 @@ struct Loader {
         const int k0 = kt * DU;
 #pragma unroll
         for (int i = 0; i < NLD; ++i) {
-            bool ok = (kidx[i] + k0) < K && voff[i] != kOOB;
-            uint32_t off = ok ? voff[i] + (uint32_t)kt * kstep : kOOB;
-            v[S][i] = buf_load16(rsrc, off);
+            if constexpr (ANY) {
+                // valid elements of the vector: K-contiguous, VE consecutive K
+                // of one row; M/N-contiguous, VE consecutive rows at one K
+                int nv;
+                if constexpr (KC) nv = voff[i] == kOOB ? 0 : min(VE, K - (kidx[i] + k0));
+                else nv = (voff[i] == kOOB || kidx[i] + k0 >= K) ? 0 : min(VE, rlim);
+                v[S][i] = gather(voff[i] + (uint32_t)kt * kstep, nv);
+            } else {
+                bool ok = (kidx[i] + k0) < K && voff[i] != kOOB;
+                uint32_t off = ok ? voff[i] + (uint32_t)kt * kstep : kOOB;
+                v[S][i] = buf_load16(rsrc, off);
+            }
         }
+    }
+
+    // The VE elements at byte offset `base`, the first nv of them loaded, the
+    // rest zero (masked loads read at kOOB, which returns zero).
+    __device__ inline i32x4_t gather(uint32_t base, int nv) const {
+        i32x4_t x;
+        if constexpr (ES == 4) {
+#pragma unroll
+            for (int e = 0; e < 4; ++e)
+                x[e] = (int)__builtin_amdgcn_raw_buffer_load_b32(rsrc, e < nv ? base + 4 * e : kOOB, 0, 0);
+        } else if constexpr (ES == 2) {
+#pragma unroll
+            for (int e = 0; e < 4; ++e) {
+                const uint32_t lo = __builtin_amdgcn_raw_buffer_load_b16(rsrc, 2 * e < nv ? base + 4 * e : kOOB, 0, 0);
+                const uint32_t hi = __builtin_amdgcn_raw_buffer_load_b16(rsrc, 2 * e + 1 < nv ? base + 4 * e + 2 : kOOB,
+                                                                         0, 0);
+                x[e] = (int)(lo | (hi << 16));
+            }
+        } else {
+#pragma unroll
+            for (int e = 0; e < 4; ++e) {
+                uint32_t w = 0;
+#pragma unroll
+                for (int q = 0; q < 4; ++q)
+                    w |= (uint32_t)__builtin_amdgcn_raw_buffer_load_b8(rsrc, 4 * e + q < nv ? base + 4 * e + q : kOOB,
+                                                                       0, 0) << (8 * q);
+                x[e] = (int)w;
+            }
+        }
+        return x;
     }
 
     // Write the K-tile in register slot S into one LDS stage (K-contiguous
@@ -272,14 +317,16 @@ __device__ inline void sched_step(int nv, int nw) {
 // Lane l of MFMA tile (ti, tj) holds the 16 consecutive rows
 // 16*(l/32) + 0..15 of column (l%32).
 // ---------------------------------------------------------------------------
-template <int OUT>
+// ANY: element by element always (the general-stride form: ldc, ldd and the
+// pointers need only their element alignment).
+template <int OUT, bool ANY = false>
 __device__ inline void store16(const TileArgs& a, const char* Cb, char* Db, int m, int n, const float* acc,
                                float alpha, float beta) {
     constexpr int OS = (OUT == kOutF32) ? 4 : 2;
     const bool use_c = (a.flags & kFlagBeta) != 0;
     char* dp = Db + ((int64_t)n * a.ldd + m) * OS;
     const char* cp = Cb + ((int64_t)n * a.ldc + m) * OS;
-    if (m + 16 <= a.M) {
+    if (!ANY && m + 16 <= a.M) {
         float r[16];
         if (use_c) {
             float c[16];
@@ -547,11 +594,20 @@ __device__ inline uint32_t extent_bytes(int64_t rows, int64_t cols_ld, int64_t t
     if (e > 0x7ffffff0ll) e = 0x7ffffff0ll;
     return (uint32_t)e;
 }
+// The same to the last element exactly (the general-stride form's element
+// loads never read past it).
+__device__ inline uint32_t extent_bytes_exact(int64_t rows, int64_t cols_ld, int64_t tail, int es) {
+    int64_t e = ((rows - 1) * cols_ld + tail) * es;
+    if (e < 0) e = 0;
+    if (e > 0x7ffffff0ll) e = 0x7ffffff0ll;
+    return (uint32_t)e;
+}
 
 // ---------------------------------------------------------------------------
 // One output tile.  smem: 2 stages x (MT_M + MT_N) x 128 bytes.
 // ---------------------------------------------------------------------------
-template <int MODE, bool AK, bool BK, int MT_M, int MT_N, int OUT>
+// ANY: the general-stride form (element-wise loads and stores, see Loader).
+template <int MODE, bool AK, bool BK, int MT_M, int MT_N, int OUT, bool ANY = false>
 __device__ inline void gemm_tile(const TileArgs& a, int tm, int tn, int b, char* smem, int tid) {
     constexpr int ES = mode_elem_bytes(MODE);
     constexpr int DU = mode_depth_u(MODE);
@@ -573,15 +629,18 @@ __device__ inline void gemm_tile(const TileArgs& a, int tm, int tn, int b, char*
     // slot each): each K-tile otherwise waits on the memory latency with only
     // the next K-tile's loads in flight.  The larger tiles lack the registers.
     constexpr int PF = (MT_M * MT_N <= 128 * 128) ? 2 : 1;
-    Loader<MODE, AK, MT_M, true, PF> la;
-    Loader<MODE, BK, MT_N, false, PF> lb;
+    Loader<MODE, AK, MT_M, true, PF, ANY> la;
+    Loader<MODE, BK, MT_N, false, PF, ANY> lb;
     // A: K-contiguous = rows M of length lda; M-contiguous = rows K of length lda
     // A K-contiguous operand's buffer starts at the tile's first row, so its
     // offsets stay within the tile's rows however large the matrix is.
-    if constexpr (AK) la.init(Ab + (int64_t)m0 * a.lda * ES, extent_bytes(a.M - m0, a.lda, a.K, ES), a.lda, a.M - m0, a.K, 0, tid);
-    else la.init(Ab, extent_bytes(a.K, a.lda, a.M, ES), a.lda, a.M, a.K, m0, tid);
-    if constexpr (BK) lb.init(Bb + (int64_t)n0 * a.ldb * ES, extent_bytes(a.N - n0, a.ldb, a.K, ES), a.ldb, a.N - n0, a.K, 0, tid);
-    else lb.init(Bb, extent_bytes(a.K, a.ldb, a.N, ES), a.ldb, a.N, a.K, n0, tid);
+    auto ext = [](int64_t rows, int64_t ld, int64_t tail) {
+        return ANY ? extent_bytes_exact(rows, ld, tail, ES) : extent_bytes(rows, ld, tail, ES);
+    };
+    if constexpr (AK) la.init(Ab + (int64_t)m0 * a.lda * ES, ext(a.M - m0, a.lda, a.K), a.lda, a.M - m0, a.K, 0, tid);
+    else la.init(Ab, ext(a.K, a.lda, a.M), a.lda, a.M, a.K, m0, tid);
+    if constexpr (BK) lb.init(Bb + (int64_t)n0 * a.ldb * ES, ext(a.N - n0, a.ldb, a.K), a.ldb, a.N - n0, a.K, 0, tid);
+    else lb.init(Bb, ext(a.K, a.ldb, a.N), a.ldb, a.N, a.K, n0, tid);
 
     f32x16_t acc[TM][TN];
 #pragma unroll
@@ -643,7 +702,7 @@ __device__ inline void gemm_tile(const TileArgs& a, int tm, int tn, int b, char*
                 la.template store<SN>(na);
                 lb.template store<SN>(na + MT_M * kRowBytes);
             }
-            sched_step<MODE, TM, TN>(rs == 0 ? la.NLD + lb.NLD : 0,
+            sched_step<MODE, TM, TN>(rs == 0 && !ANY ? la.NLD + lb.NLD : 0,
                                      rs + 2 == RS ? la.NSTORE + lb.NSTORE : 0);
         }
         lds_barrier();
@@ -687,7 +746,7 @@ __device__ inline void gemm_tile(const TileArgs& a, int tm, int tn, int b, char*
                 float v[16];
 #pragma unroll
                 for (int e = 0; e < 16; ++e) v[e] = acc[i][j][e];
-                store16<OUT>(a, Cb, Db, m, n, v, alpha, a.beta);
+                store16<OUT, ANY>(a, Cb, Db, m, n, v, alpha, a.beta);
             }
         }
     }
@@ -719,6 +778,16 @@ __global__ void __launch_bounds__(kThreads, 1) gemm_kernel(TileArgs a) {
     int tb, tm, tn;
     map_tile(blockIdx.x, gridDim.x, a, tb, tm, tn);
     gemm_tile<MODE, AK, BK, MT_M, MT_N, OUT>(a, tm, tn, tb, smem, threadIdx.x);
+}
+
+// The general-stride form: any leading dimension, batch stride and K, pointers
+// aligned to their element size; MT x MT tiles.
+template <int MODE, bool AK, bool BK, int MT, int OUT>
+__global__ void __launch_bounds__(kThreads, 1) gemm_kernel_any(TileArgs a) {
+    __shared__ __attribute__((aligned(16))) char smem[2 * (MT + MT) * kRowBytes];
+    int tb, tm, tn;
+    map_tile(blockIdx.x, gridDim.x, a, tb, tm, tn);
+    gemm_tile<MODE, AK, BK, MT, MT, OUT, true>(a, tm, tn, tb, smem, threadIdx.x);
 }
 
 // ---------------------------------------------------------------------------

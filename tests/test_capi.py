@@ -73,9 +73,7 @@ def test_query_tile_small_shapes_pick_small_tiles():
 
 
 @pytest.mark.parametrize("bad, msg", [
-    (dict(K=500, layout="TN", lda=504, ldb=504), "multiple of 8"),   # K-contiguous operand, K % 8
     (dict(lda=100), "lda"),                                # lda < M
-    (dict(ldd=516), "multiples"),                          # 16-byte rows of D
     (dict(layout="NN", dtype=C.F8E4M3), "TN"),             # fp8 is TN only
     (dict(compute=C.COMPUTE_BF16X3), "fp32 operands"),     # bf16x3 needs fp32
     (dict(dtype=9), "dtype"),
@@ -110,6 +108,73 @@ def test_operand_size_limits():
     # bf16x3 planes are K-contiguous too: a workspace is offered past 2 GiB of planes
     x3 = tl.make_problem("TN", C.F32, M, N, K, K, K, M, M, compute=C.COMPUTE_BF16X3)
     assert tl.workspace_size(x3) == ((M * 3 * K * 2 + 255) // 256 + (N * 3 * K * 2 + 255) // 256) * 256
+
+
+@pytest.mark.parametrize("kw, mt", [
+    (dict(K=500, layout="TN", lda=504, ldb=504), 64),      # K-contiguous operand, K % 8
+    (dict(ldd=513), 64),                                   # D rows off the 16-byte grid
+    (dict(lda=4097, M=4096, N=4096), 128),                 # enough 128 x 128 tiles for the CUs
+    (dict(dtype=C.F32, ldb=513), 64),
+    (dict(dtype=C.F32, compute=C.COMPUTE_BF16X3, lda=515), 64),
+    (dict(dtype=C.F8E4M3, layout="TN", K=200, lda=200, ldb=200, dtype_out=C.F32), 128),
+])
+def test_general_stride_plan(kw, mt):
+    """Leading dimensions, K or batch strides off the 16-byte grid run the
+    general-stride kernel (element-wise loads and stores): a square 64 x 64 or
+    128 x 128 tile (fp8: 128), 2 x 2 waves, two LDS stages, never split."""
+    kw = dict(kw)
+    layout = kw.pop("layout", "NN")
+    dtype = kw.pop("dtype", C.BF16)
+    M, N, K = kw.pop("M", 512), kw.pop("N", 512), kw.pop("K", 512)
+    p = _p(layout, dtype, M, N, K, **kw)
+    t = tl.query_tile(p)
+    assert (t.mt_m, t.mt_n, t.waves_m, t.waves_n, t.lds_buffers, t.mfma_mn) == (mt, mt, 2, 2, 2, 32)
+    assert tl.query_tile_ws(p).split_k == 1
+    # a record of the form runs as given; any other is refused with the reason
+    assert tl.query_tile(p, (64, 64) if dtype != C.F8E4M3 else (128, 128)).mt_m in (64, 128)
+    with pytest.raises(tl.FFGemmError) as e:
+        tl.query_tile(p, (256, 256))
+    assert e.value.status == C.ERR and "general-stride" in str(e.value)
+    with pytest.raises(tl.FFGemmError):
+        tl.query_tile(p, (128, 128, 0, 0, 0, 0, 2))    # no split
+    with pytest.raises(tl.FFGemmError):
+        tl.query_tile(p, (64, 64, 0, 0, 0, 16))        # no 16 x 16 MFMA form
+
+
+def test_tile_from_reference_off_grid_problem():
+    """A reference record whose tile the general-stride kernel lacks runs as
+    the library's choice for the problem, with RTN_WARN saying so."""
+    p = tl.make_problem("TN", C.BF16, 512, 512, 512, 513, 513, 513, 513)
+    st, t, msg = tl.tile_from_reference(p, 4, 2, 2, 4, 32)      # MT 256 x 256, 2 x 4 waves
+    assert st == C.WARN and (t.mt_m, t.mt_n, t.waves_m, t.waves_n) == (64, 64, 2, 2)
+    assert "runs as MT64x64" in msg
+    st, t, _ = tl.tile_from_reference(p, 2, 2, 2, 2, 32)        # MT 128 x 128, 2 x 2 waves: exact
+    assert st == C.SUCCESS and (t.mt_m, t.mt_n) == (128, 128)
+
+
+def test_general_stride_pointer_alignment():
+    """The general-stride form needs pointers aligned to their element size: a
+    bf16 matrix at an odd byte address is refused before any launch."""
+    p = _p("NN", C.BF16, 64, 64, 64, lda=65)
+    fake = ctypes.c_void_p(0x100000)
+    odd = ctypes.c_void_p(0x100001)
+    st = C.lib().ffgemm_run(ctypes.byref(p), odd, fake, None, fake, None, None)
+    assert st == C.ERR and "element size" in C.lib().ffgemm_last_error().decode()
+
+
+def test_grouped_plan_rejects_off_grid_problems():
+    """Grouped plans run the aligned engines only: a member off the 16-byte
+    grid is refused at create, before anything is allocated."""
+    p = _p("NN", C.BF16, 256, 256, 256, lda=257)
+    probs = (C.Problem * 2)(p, p)
+    fake = ctypes.c_void_p(0x100000)
+    arr = (ctypes.c_void_p * 2)(fake, fake)
+    plan = ctypes.c_void_p(0)
+    t = C.Tile()
+    t.mt_m = t.mt_n = 64
+    st = C.lib().ffgemm_grouped_plan_create(ctypes.byref(plan), 2, probs, arr, arr, None, arr, ctypes.byref(t))
+    assert st == C.ERR and plan.value is None
+    assert "16-byte grid" in C.lib().ffgemm_last_error().decode()
 
 
 def test_bad_tile_request_is_rejected():

@@ -52,6 +52,14 @@ def _perf(stdout: str) -> float:
     # the reference's tile flags asking for 256 x 256 with 2 x 4 waves (the eight-wave engine)
     ["-d", "3", "-m", "1024", "-n", "1024", "-k", "1024", "-r", "4", "-s", "2", "-x", "2", "-y", "4", "-z", "32",
      "--init", "normal"],
+    # odd padding (ld = rows + pad off the 16-byte grid) and K off the vector:
+    # the general-stride kernel, with the record's tile or, for 256 x 256, the
+    # library's choice (a warning)
+    ["-d", "1", "-m", "480", "-n", "512", "-k", "512", "-p", "1"],
+    ["-d", "2", "-m", "768", "-n", "640", "-k", "256", "-p", "3", "-r", "2", "-s", "2", "-x", "2", "-y", "2"],
+    ["-d", "3", "-m", "480", "-n", "512", "-k", "100", "-p", "1", "--init", "normal"],
+    ["-d", "3", "-m", "512", "-n", "512", "-k", "512", "-p", "1", "-r", "4", "-s", "2", "-x", "2", "-y", "4",
+     "-z", "32", "--init", "normal"],
 ])
 def test_cli_run_verifies_and_reports(cuda, args):
     _need_bin()
