@@ -90,8 +90,9 @@ typedef struct ffgemm_tile {
     int32_t depth_u;  /* K per main-loop iteration (fixed by dtype: 128 bytes of K) */
     int32_t waves_m;  /* waves along M (2) */
     int32_t waves_n;  /* waves along N (2) */
-    int32_t mfma_mn;  /* MFMA M/N width: 32 (default), or 16 for 16x16x32 MFMAs
-                       * (bf16/fp16 at 64x64 and 128x128; -z of the reference) */
+    int32_t mfma_mn;  /* MFMA M/N width: 32 (default), or 16 for 16x16 MFMAs
+                       * (16x16x32 bf16/fp16, 16x16x4 fp32; at 64x64 and
+                       * 128x128; -z of the reference) */
     int32_t group_m;  /* tile-raster group height (0 = default) */
     int32_t lds_buffers; /* 2 */
     int32_t split_k;  /* K slices (split-K), summed by a reduction pass: 0 = the

@@ -407,12 +407,12 @@ int prepare_with(const ffgemm_problem* p, const void* A, const void* B, const vo
                 return fail(FFGEMM_ERR, "no %dx%d-wave kernel for MT%dx%d with this dtype and layout",
                             tile->waves_m, tile->waves_n, tile->mt_m, tile->mt_n);
         }
-        // mfma_mn: 0 or 32 (32 x 32 MFMAs), or 16 (16 x 16 x 32, bf16 / fp16 at
-        // 64 x 64 and 128 x 128)
+        // mfma_mn: 0 or 32 (32 x 32 MFMAs), or 16 (16 x 16 x 32 for bf16 / fp16,
+        // 16 x 16 x 4 for fp32, at 64 x 64 and 128 x 128)
         if (tile->mfma_mn == 16) {
             if (engine_for(mode, ta, !tb, tile->mt_m, tile->mt_n, waves, 16) == kEngNone)
-                return fail(FFGEMM_ERR, "no 16x16 MFMA kernel for MT%dx%d with this dtype (bf16/fp16 at 64x64, 128x128)",
-                            tile->mt_m, tile->mt_n);
+                return fail(FFGEMM_ERR, "no 16x16 MFMA kernel for MT%dx%d with this dtype (bf16/fp16/fp32 at 64x64, "
+                            "128x128)", tile->mt_m, tile->mt_n);
             mfma = 16;
         } else if (tile->mfma_mn && tile->mfma_mn != 32) {
             return fail(FFGEMM_ERR, "unsupported mfma_mn %d (16 or 32)", tile->mfma_mn);

@@ -20,14 +20,16 @@ namespace ffgemm {
 // engine (gemm_tile.hpp) for the rest and for fp32 / bf16x3.  waves = 8 or 4 asks for the eight-wave or a
 // four-wave engine (the wave-tiling knob of the tile record, which the
 // autotuner sweeps); kEngNone when the combination has no kernel.  mfma = 16
-// asks for 16 x 16 x 32 MFMAs (the MFMA-instruction knob): the first engine's
-// 16 x 16 form (gemm_tile_m16) at 64 x 64 and 128 x 128 for bf16 / fp16.
+// asks for 16 x 16 MFMAs (the MFMA-instruction knob): the first engine's
+// 16 x 16 form (gemm_tile_m16) at 64 x 64 and 128 x 128 for bf16 / fp16
+// (16 x 16 x 32) and fp32 (16 x 16 x 4).
 enum Engine : int { kEngNone = 0, kEngV1 = 1, kEngV2 = 2, kEngV3 = 3, kEngV4 = 4, kEngV1M16 = 5 };
 __host__ __device__ constexpr Engine engine_for(int mode, bool ak, bool bk, int tm, int tn, int waves = 0,
                                                 int mfma = 0) {
     const bool k16 = (mode == kBF16 || mode == kF16);
     if (mfma == 16)
-        return (k16 && (waves == 0 || waves == 4) && tm == tn && (tm == 64 || tm == 128)) ? kEngV1M16 : kEngNone;
+        return ((k16 || mode == kF32) && (waves == 0 || waves == 4) && tm == tn && (tm == 64 || tm == 128))
+                   ? kEngV1M16 : kEngNone;
     if (mfma != 0 && mfma != 32) return kEngNone;
     const bool big = (tm == 256 && tn == 256);
     const bool v4ok = big && (k16 || (mode == kF8 && ak && bk));

@@ -791,33 +791,40 @@ __global__ void __launch_bounds__(kThreads, 1) gemm_kernel_any(TileArgs a) {
 }
 
 // ---------------------------------------------------------------------------
-// The same tile with 16 x 16 x 32 MFMAs (tile record mfma_mn = 16; bf16 and
-// fp16).  Lane l supplies row (column) l % 16 of its operand block and the 8
-// consecutive K of 16-byte chunk l / 16 of the read step, so one MFMA covers
-// 32 K and a 128-byte K-tile takes two read steps; the result block gives lane
-// l rows 4 * (l / 16) .. +3 of column l % 16.  Those rows are consecutive in
-// the natural row order, so A is staged unpermuted.  The pipeline is the
-// 32 x 32 tile's with two read steps and two K-tiles of loads in flight.
+// The same tile with 16 x 16 MFMAs (tile record mfma_mn = 16): 16 x 16 x 32 for
+// bf16 and fp16, 16 x 16 x 4 for fp32.  Lane l supplies row (column) l % 16 of
+// its operand block from 16-byte chunk l / 16 of the read step: 8 consecutive
+// K of 16-bit data (one MFMA covers 32 K), or 4 fp32 (four MFMAs, element e of
+// every lane's chunk in MFMA e: 16 K).  Either way a 128-byte K-tile takes two
+// read steps, and A and B lanes pair the same K.  The result block gives lane l
+// rows 4 * (l / 16) .. +3 of column l % 16.  Those rows are consecutive in the
+// natural row order, so A is staged unpermuted.  The pipeline is the 32 x 32
+// tile's with two read steps and two K-tiles of loads in flight.
 // ---------------------------------------------------------------------------
 template <int MODE>
 __device__ inline void mfma16x16(const i32x4_t& a, const i32x4_t& b, f32x4_t& acc) {
-    if constexpr (MODE == kBF16)
+    if constexpr (MODE == kBF16) {
         acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(__builtin_bit_cast(bf16x8_t, a), __builtin_bit_cast(bf16x8_t, b),
                                                       acc, 0, 0, 0);
-    else
+    } else if constexpr (MODE == kF16) {
         acc = __builtin_amdgcn_mfma_f32_16x16x32_f16(__builtin_bit_cast(f16x8_t, a), __builtin_bit_cast(f16x8_t, b),
                                                      acc, 0, 0, 0);
+    } else {
+#pragma unroll
+        for (int e = 0; e < 4; ++e)
+            acc = __builtin_amdgcn_mfma_f32_16x16x4f32(as_float(a[e]), as_float(b[e]), acc, 0, 0, 0);
+    }
 }
 
 template <int MODE, bool AK, bool BK, int MT_M, int MT_N, int OUT>
 __device__ inline void gemm_tile_m16(const TileArgs& a, int tm, int tn, int b, char* smem, int tid) {
-    static_assert(MODE == kBF16 || MODE == kF16, "16 x 16 x 32 MFMA: 16-bit operands");
-    constexpr int ES = 2;
+    static_assert(MODE == kBF16 || MODE == kF16 || MODE == kF32, "16 x 16 MFMA: 16-bit or fp32 operands");
+    constexpr int ES = mode_elem_bytes(MODE);
     constexpr int DU = mode_depth_u(MODE);
     constexpr int WT_M = MT_M / 2, WT_N = MT_N / 2;
     constexpr int TM = WT_M / 16, TN = WT_N / 16;
     constexpr int STAGE = (MT_M + MT_N) * kRowBytes;
-    static_assert(kRowBytes == 128 && DU == 64, "two read steps of 32 K per K-tile");
+    static_assert(kRowBytes == 128, "two read steps of 64 bytes of K per K-tile");
 
     const int lane = tid & 63;
     const int wave = tid >> 6;

@@ -294,15 +294,17 @@ def test_wave_tiling_knob_rejects(layout, dtype, tile, waves):
 
 
 @pytest.mark.parametrize("layout", ["NN", "NT", "TN", "TT"])
-@pytest.mark.parametrize("dtype", [C.BF16, C.F16])
+@pytest.mark.parametrize("dtype", [C.BF16, C.F16, C.F32])
 @pytest.mark.parametrize("mt", [64, 128])
 def test_mfma_knob_16x16(layout, dtype, mt):
-    """mfma_mn = 16 selects the 16 x 16 x 32 MFMA form of the first engine
-    (four waves of 2 x 2, K = 64 per stage, two LDS stages) and is reported back."""
+    """mfma_mn = 16 selects the 16 x 16 MFMA form of the first engine (16 x 16
+    x 32 for 16-bit operands, 16 x 16 x 4 for fp32; four waves of 2 x 2, 128
+    bytes of K per stage, two LDS stages) and is reported back."""
     t = C.Tile()
     t.mt_m, t.mt_n, t.mfma_mn = mt, mt, 16
     r = tl.query_tile(_p(layout, dtype, 1000, 520, 328), t)
-    assert (r.mt_m, r.mt_n, r.mfma_mn, r.waves_m, r.waves_n, r.depth_u, r.lds_buffers) == (mt, mt, 16, 2, 2, 64, 2)
+    du = 32 if dtype == C.F32 else 64
+    assert (r.mt_m, r.mt_n, r.mfma_mn, r.waves_m, r.waves_n, r.depth_u, r.lds_buffers) == (mt, mt, 16, 2, 2, du, 2)
     t32 = C.Tile()
     t32.mt_m, t32.mt_n, t32.mfma_mn = mt, mt, 32
     assert tl.query_tile(_p(layout, dtype, 1000, 520, 328), t32).mfma_mn == 32
@@ -311,7 +313,6 @@ def test_mfma_knob_16x16(layout, dtype, mt):
 @pytest.mark.parametrize("dtype,tile,mfma,waves", [
     (C.BF16, (256, 256), 16, (0, 0)),  # 16 x 16 only at 64 x 64 and 128 x 128
     (C.BF16, (128, 256), 16, (0, 0)),
-    (C.F32, (64, 64), 16, (0, 0)),     # 16-bit operands only
     (C.BF16, (64, 64), 16, (2, 4)),    # four waves only
     (C.BF16, (64, 64), 8, (0, 0)),     # no such instruction width
 ])
@@ -320,6 +321,15 @@ def test_mfma_knob_rejects(dtype, tile, mfma, waves):
     t.mt_m, t.mt_n, t.mfma_mn, t.waves_m, t.waves_n = tile[0], tile[1], mfma, waves[0], waves[1]
     with pytest.raises(C.FFGemmError, match="RTN_ERR"):
         tl.query_tile(_p("NN", dtype, 1000, 520, 328), t)
+
+
+def test_mfma_knob_16x16_not_for_bf16x3_or_fp8():
+    """The 16 x 16 form takes bf16, fp16 and fp32 operands; bf16x3 and fp8 run
+    32 x 32 MFMAs only."""
+    for p in (_p("NN", C.F32, 1000, 520, 328, compute=C.COMPUTE_BF16X3),
+              _p("TN", C.F8E4M3, 1000, 520, 320, dtype_out=C.F32)):
+        with pytest.raises(C.FFGemmError, match="16x16"):
+            tl.query_tile(p, (64, 64, 0, 0, 0, 16))
 
 
 def test_tuned_tile_lookup(tmp_path, monkeypatch):

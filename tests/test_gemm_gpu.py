@@ -808,6 +808,33 @@ def test_16x16_mfma_form(cuda, name, layout, mt):
 
 @pytest.mark.parametrize("layout", LAYOUTS)
 @pytest.mark.parametrize("mt", [64, 128])
+def test_16x16_mfma_form_fp32(cuda, layout, mt):
+    """fp32 operands on 16 x 16 x 4 MFMAs (mfma_mn = 16): a partial K-tile,
+    ragged M and N, padded leading dimensions, alpha / beta with C, against
+    the FP64 oracle (rel-Frobenius <= 1e-5); D's padding rows untouched."""
+    M, N, K = 330, 270, 212
+    _say(f"case 16x16 MFMA fp32 {layout} tile={mt}x{mt}")
+    rng = np.random.default_rng(23)
+    (ra, ca), (rb, cb) = op_dims(layout, M, N, K)
+    lda, ldb, ldd = (ra + 3) // 4 * 4 + 4, (rb + 3) // 4 * 4, (M + 3) // 4 * 4 + 8
+    A = colmajor(rng, ra, ca, lda, kind="normal")
+    B = colmajor(rng, rb, cb, ldb, kind="normal")
+    Cm = colmajor(rng, M, N, ldd)
+    Dd = torch.full((ldd * N,), 7.0, device=cuda)
+    p = tl.make_problem(layout, C.F32, M, N, K, lda, ldb, ldd, ldd, 0.5, -1.5)
+    t = tl.query_tile(p, (mt, mt, 0, 0, 0, 16))
+    assert (t.mt_m, t.mfma_mn, t.depth_u) == (mt, 16, 32)
+    tl.run(p, torch.from_numpy(A).to(cuda), torch.from_numpy(B).to(cuda), torch.from_numpy(Cm).to(cuda), Dd,
+           tile=(mt, mt, 0, 0, 0, 16), workspace=None)
+    torch.cuda.synchronize()
+    got = Dd.cpu().numpy()
+    ref = _f64_ref(layout, M, N, K, A, lda, B, ldb, 0.5, -1.5, Cm, ldd, ldd)
+    assert rel_fro(extract(got, M, N, ldd), extract(ref, M, N, ldd)) <= 1e-5
+    assert np.all(got.reshape(N, ldd)[:, M:] == 7.0)
+
+
+@pytest.mark.parametrize("layout", LAYOUTS)
+@pytest.mark.parametrize("mt", [64, 128])
 def test_16x16_mfma_form_f32_out(cuda, layout, mt):
     """bf16 inputs, fp32 D and C through the 16 x 16 MFMA form."""
     M, N, K = 144, 104, 264

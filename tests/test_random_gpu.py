@@ -123,7 +123,9 @@ def test_random_fp32_forms(cuda, seed, form):
         M, N = int(rng.integers(1, 500)), int(rng.integers(1, 500))
         K = int(rng.integers(1, 120)) * 4         # K-contiguous fp32 operands need K % 4 == 0
         beta = [0.0, 1.0, -0.5][rng.integers(3)]
-        tile = _F32_TILES[rng.integers(len(_F32_TILES))]
+        # fp32 MFMA also on its 16 x 16 x 4 form (mfma_mn = 16)
+        tiles = _F32_TILES + ([(64, 64, 0, 0, 0, 16), (128, 128, 0, 0, 0, 16)] if form == "f32" else [])
+        tile = tiles[rng.integers(len(tiles))]
         print(f"case {form} seed={seed}.{case} {layout} M={M} N={N} K={K} b={beta} tile={tile}", flush=True)
         (ra, ca), (rb, cb) = op_dims(layout, M, N, K)
         lda, ldb, ldd = (ra + 3) // 4 * 4, (rb + 3) // 4 * 4, (M + 3) // 4 * 4
