@@ -236,15 +236,18 @@ def main(argv=None):
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--group-ms", default="0", help="tile-raster group heights to try, e.g. 0,2,8 (0: default)")
     ap.add_argument("--set", default="bench", choices=("bench", "phantom", "all"),
-                    help="bench: the benchmark's problems and common shapes; phantom: the reference's thin "
-                         "shapes (test/phantom_test.sh) with split-K candidates")
+                    help="bench: the benchmark's problems and common shapes (split-K candidates where 256 x 256 "
+                         "tiles leave CUs idle); phantom: the reference's thin shapes (test/phantom_test.sh) with "
+                         "split-K candidates")
     a = ap.parse_args(argv)
     gms = tuple(int(x) for x in a.group_ms.split(","))
     dev = torch.device("cuda", torch.cuda.current_device())
     db = load_db(a.db)
     todo = []
     if a.set in ("bench", "all"):
-        todo += [(pr, (1,)) for pr in TUNE_PROBLEMS]
+        # split-K candidates where 256 x 256 tiles leave CUs idle
+        ncu = torch.cuda.get_device_properties(dev).multi_processor_count
+        todo += [(pr, SPLITS if -(-pr[2] // 256) * -(-pr[3] // 256) < ncu else (1,)) for pr in TUNE_PROBLEMS]
     if a.set in ("phantom", "all"):
         from . import phantom
         code = {"f32": C.F32, "bf16": C.BF16, "f16": C.F16}
