@@ -169,7 +169,13 @@ int ffgemm_run(const ffgemm_problem *p, const void *A, const void *B, const void
  *  - split-K: a single problem with too few output tiles to fill the CUs
  *    (by the library's cost model, or a tile record with split_k >= 2) cuts K
  *    into S slices whose fp32 partial products (S planes of roundup(M, 4) x N)
- *    are summed by a reduction pass; without a workspace it runs unsplit. */
+ *    are summed by a reduction pass; without a workspace it runs unsplit;
+ *  - tail split: with no tile given (and no tuning-table entry), a problem
+ *    whose 256 x 256 tiles fill whole passes over the CUs plus a short last
+ *    pass runs as two sub-problems, the whole passes as one launch and the
+ *    remaining tile columns (or rows) with split-K; the workspace holds the
+ *    remainder's planes.  ffgemm_query_tile_ws then still reports the
+ *    single-launch record. */
 int ffgemm_workspace_size(const ffgemm_problem *p, size_t *bytes);
 
 /* The tile record ffgemm_run_ws follows for this problem given a workspace of

@@ -817,6 +817,112 @@ int run_split(const ffgemm_problem* p, const void* A, const void* B, const void*
     hipError_t e = hipGetLastError();
     return e == hipSuccess ? FFGEMM_SUCCESS : hip_fail(e, "split-K reduce launch");
 }
+
+// Tail split: a problem whose 256 x 256 tiles fill r >= 1 whole passes over
+// the CUs plus a short last pass runs as two sub-problems on the workspace
+// path, the data-parallel part (whole 256 x 256 tile columns, or rows, that
+// fill the r passes) and the remainder with split-K, so the last pass runs S
+// times shorter slices on S times more CUs instead of leaving most CUs idle.
+// Taken when the cost model puts it 15 % ahead of the single launch the
+// library would otherwise make (its own tile choice, no split).
+struct TailPlan {
+    bool along_n;  // cut between tile columns (else tile rows)
+    int64_t cut;   // first column (row) of the remainder
+    int S;         // K slices of the remainder
+};
+
+// The sub-problem of rows [m0, m1) and columns [n0, n1) of D, with the
+// operand, C and D pointers offset to it.
+void sub_problem(const ffgemm_problem* p, const void*& A, const void*& B, const void*& C, void*& D, int64_t m0,
+                 int64_t m1, int64_t n0, int64_t n1, ffgemm_problem& q) {
+    const bool ta = p->layout >= FFGEMM_TN, tb = (p->layout & 1) != 0;
+    const int es = p->dtype == FFGEMM_F32 ? 4 : p->dtype == FFGEMM_F8E4M3 ? 1 : 2;
+    const int out = out_kind_of(p->dtype, p->dtype_out);
+    const int os = out == kOutF32 ? 4 : 2;
+    q = *p;
+    q.M = m1 - m0;
+    q.N = n1 - n0;
+    A = (const char*)A + (ta ? m0 * p->lda : m0) * es;   // op(A) row m0
+    B = (const char*)B + (tb ? n0 : n0 * p->ldb) * es;   // op(B) column n0
+    if (C) C = (const char*)C + (n0 * p->ldc + m0) * os;
+    D = (char*)D + (n0 * p->ldd + m0) * os;
+}
+
+bool tail_plan(const ffgemm_problem* p, const Launch& L, TailPlan& tp) {
+    if (L.split != 1 || L.any || L.grid <= 0 || p->batch > 1 || L.waves || L.mfma) return false;
+    if (L.mode != kBF16 && L.mode != kF16 && L.mode != kF8) return false;
+    if (mn_k_chunk(p)) return false;
+    const int64_t M = p->M, N = p->N, K = p->K, G = cu_count();
+    const int64_t tm = cdiv(M, 256), tn = cdiv(N, 256), tiles = tm * tn, rounds = tiles / G;
+    if (rounds < 1 || tiles % G == 0) return false;
+    const int osz = L.out == kOutF32 ? 4 : 2;
+    const bool use_c = p->beta != 0.0f;
+    int ti = 0;  // the single launch's tile, as planned
+    for (int i = 0; i < 5; ++i)
+        if (kTiles[i].m == L.tile.m && kTiles[i].n == L.tile.n) ti = i;
+    const double full = plan_cost(M, N, K, L.mode, osz, use_c, ti, 1);
+    double best = full * 0.85;
+    bool found = false;
+    static const int cand[] = {2, 3, 4, 6, 8};
+    for (int along_n = 1; along_n >= 0; --along_n) {
+        const int64_t whole = along_n ? std::min(tn - 1, rounds * G / tm) : std::min(tm - 1, rounds * G / tn);
+        if (whole < 1) continue;
+        const int64_t cut = whole * 256;
+        const double dp = along_n ? plan_cost(M, cut, K, L.mode, osz, use_c, 0, 1)
+                                  : plan_cost(cut, N, K, L.mode, osz, use_c, 0, 1);
+        for (int S : cand) {
+            if (K / S < split_min_k(L.mode)) break;
+            const double c = dp + (along_n ? plan_cost(M, N - cut, K, L.mode, osz, use_c, 0, S)
+                                           : plan_cost(M - cut, N, K, L.mode, osz, use_c, 0, S));
+            if (c < best) { best = c; tp = TailPlan{along_n != 0, cut, S}; found = true; }
+        }
+    }
+    return found;
+}
+
+// The tail plan ffgemm_run_ws follows for p (no caller tile, no tuning-table
+// entry), with the remainder's launch plan; false when it runs as one plan.
+bool tail_plan_for(const ffgemm_problem* p, const ffgemm_tile* tile, const Launch& L, TailPlan& tp, Launch& LT) {
+    ffgemm_tile tuned;
+    if ((tile && tile->mt_m) || tuned_tile(p, &tuned) || !tail_plan(p, L, tp)) return false;
+    static const int64_t dummy[4] = {0, 0, 0, 0};
+    const void *A = dummy, *B = dummy, *C = dummy;
+    void* D = (void*)dummy;
+    ffgemm_problem q;
+    if (tp.along_n) sub_problem(p, A, B, C, D, 0, p->M, tp.cut, p->N, q);
+    else sub_problem(p, A, B, C, D, tp.cut, p->M, 0, p->N, q);
+    ffgemm_tile t;
+    memset(&t, 0, sizeof(t));
+    t.mt_m = t.mt_n = 256;
+    t.split_k = tp.S;
+    return prepare_with(&q, dummy, dummy, dummy, (void*)dummy, &t, LT, true) == FFGEMM_SUCCESS && LT.split > 1;
+}
+
+// Runs a tail plan: the data-parallel part, then the remainder with split-K.
+int run_tail(const ffgemm_problem* p, const void* A, const void* B, const void* C, void* D, const TailPlan& tp,
+             float* ws, hipStream_t s) {
+    ffgemm_problem q0, q1;
+    const void *A0 = A, *B0 = B, *C0 = C, *A1 = A, *B1 = B, *C1 = C;
+    void *D0 = D, *D1 = D;
+    if (tp.along_n) {
+        sub_problem(p, A0, B0, C0, D0, 0, p->M, 0, tp.cut, q0);
+        sub_problem(p, A1, B1, C1, D1, 0, p->M, tp.cut, p->N, q1);
+    } else {
+        sub_problem(p, A0, B0, C0, D0, 0, tp.cut, 0, p->N, q0);
+        sub_problem(p, A1, B1, C1, D1, tp.cut, p->M, 0, p->N, q1);
+    }
+    ffgemm_tile t;
+    memset(&t, 0, sizeof(t));
+    t.mt_m = t.mt_n = 256;
+    t.split_k = 1;
+    int r = ffgemm_run(&q0, A0, B0, C0, D0, &t, s);
+    if (r != FFGEMM_SUCCESS) return r;
+    Launch LT;
+    t.split_k = tp.S;
+    r = prepare_with(&q1, A1, B1, C1, D1, &t, LT, true);
+    if (r != FFGEMM_SUCCESS) return r;
+    return run_split(&q1, A1, B1, C1, D1, LT, ws, s);
+}
 }  // namespace
 
 extern "C" {
@@ -832,9 +938,12 @@ int ffgemm_workspace_size(const ffgemm_problem* p, size_t* bytes) {
     // (validation with dummy aligned pointers; nothing is dereferenced)
     static const int64_t dummy[4] = {0, 0, 0, 0};
     Launch L;
-    if (p->dtype != FFGEMM_F64 && split_plan(p, dummy, dummy, dummy, (void*)dummy, nullptr, L) == FFGEMM_SUCCESS &&
-        L.split > 1)
-        *bytes = (size_t)split_ws_bytes(L);
+    if (p->dtype != FFGEMM_F64 && split_plan(p, dummy, dummy, dummy, (void*)dummy, nullptr, L) == FFGEMM_SUCCESS) {
+        TailPlan tp;
+        Launch LT;
+        if (L.split > 1) *bytes = (size_t)split_ws_bytes(L);
+        else if (tail_plan_for(p, nullptr, L, tp, LT)) *bytes = (size_t)split_ws_bytes(LT);  // the remainder's planes
+    }
     return FFGEMM_SUCCESS;
 }
 
@@ -876,6 +985,13 @@ int ffgemm_run_ws(const ffgemm_problem* p, const void* A, const void* B, const v
             ffgemm_tile t = *tile;
             t.split_k = 1;
             return ffgemm_run(p, A, B, C, D, &t, stream);
+        }
+        TailPlan tp;
+        Launch LT;
+        if (L.split <= 1 && tail_plan_for(p, tile, L, tp, LT) && workspace &&
+            workspace_bytes >= (size_t)split_ws_bytes(LT)) {
+            if ((uintptr_t)workspace % 256) return fail(FFGEMM_ERR, "workspace must be 256-byte aligned");
+            return run_tail(p, A, B, C, D, tp, (float*)workspace, (hipStream_t)stream);
         }
         return ffgemm_run(p, A, B, C, D, tile, stream);
     }

@@ -527,6 +527,27 @@ def test_split_k_planning():
     assert (t.mt_m, t.mt_n, t.split_k) == (128, 128, 3)
 
 
+def test_tail_split_planning():
+    """With no tile on the workspace path, a problem whose 256 x 256 tiles fill
+    whole passes over the CUs plus a short last pass asks for the remainder's
+    split-K planes (the data-parallel part needs none); one that fills whole
+    passes, or whose last pass is long, asks for nothing.  ffgemm_query_tile_ws
+    reports the single-launch record, and a caller's tile (or a tuning-table
+    entry) runs as given."""
+    cus = C.lib().ffgemm_device_cu_count()
+    if cus != 256:
+        pytest.skip(f"planning figures assume 256 CUs ({cus} here)")
+    for M, N, K, want in [(6144, 6144, 6144, True), (4608, 4608, 4608, True), (7168, 7168, 7168, True),
+                          (69120, 256, 2048, True), (4096, 4096, 4096, False), (8192, 8192, 8192, False),
+                          (5120, 5120, 5120, False), (6144, 6144, 2048, False)]:
+        p = _p("TN", C.BF16, M, N, K)
+        assert (tl.workspace_size(p) > 0) == want, (M, N, K)
+        assert tl.query_tile_ws(p).split_k == 1
+    # 4608^3: 18 x 18 tiles, 252 in one pass, the remaining 4 tile columns (72 tiles) in 3 slices
+    p = _p("TN", C.BF16, 4608, 4608, 4608)
+    assert tl.workspace_size(p) == 3 * 4608 * 1024 * 4
+
+
 @pytest.mark.parametrize("case", ["run", "batch", "x3", "short_k", "negative"])
 def test_split_k_rejections(case):
     """A split the problem cannot take is refused with RTN_ERR: through

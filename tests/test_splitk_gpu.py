@@ -167,3 +167,34 @@ def test_split_k_graph_replay(cuda):
         torch.cuda.synchronize()
         assert torch.equal(D, D0), i
         (A * 2).sum().item()   # eager work between replays
+
+
+@pytest.mark.parametrize("name,shape,beta", [("bf16", (4608, 4608, 4608), 0.5), ("f16", (4608, 4608, 4608), 0.0),
+                                             ("bf16", (69120, 256, 2048), 1.0)])
+def test_tail_split(cuda, name, shape, beta):
+    """Problems whose 256 x 256 tiles leave a short last pass run, on the
+    workspace path with no tile given, as a data-parallel part plus a split-K
+    remainder (cut between tile columns for the square shape, between tile rows
+    for the tall one).  Sampled output columns (both parts; every row) against
+    the FP64 oracle, alpha / beta with C."""
+    M, N, K = shape
+    code = C.BF16 if name == "bf16" else C.F16
+    tdt = torch.bfloat16 if name == "bf16" else torch.float16
+    p = tl.make_problem("TN", code, M, N, K, K, K, M, M, 1.25, beta)
+    assert tl.workspace_size(p) > 0 and tl.query_tile_ws(p).split_k == 1
+    _say(f"case tail split {name} {shape} beta={beta}")
+    g = torch.Generator(device="cpu").manual_seed(M + K)
+    A = (torch.randn(M * K, generator=g) / np.sqrt(K)).to(tdt)
+    B = (torch.randn(N * K, generator=g) / np.sqrt(K)).to(tdt)
+    Cm = torch.randn(M * N, generator=g).to(tdt)
+    D = torch.full((M * N,), float("nan"), dtype=tdt, device=cuda)
+    tl.run(p, A.to(cuda), B.to(cuda), Cm.to(cuda), D)
+    torch.cuda.synchronize()
+    rng = np.random.default_rng(N)
+    cols = np.unique(np.concatenate([rng.choice(N, 3, replace=False), [N - 1, 0]]))
+    Bc = B.double().numpy().reshape(N, K)[cols]
+    ref = oracle.gemm_f64("TN", M, len(cols), K, 1.25, A.double().numpy(), K, Bc.ravel(), K).reshape(len(cols), M)
+    ref += beta * Cm.double().numpy().reshape(N, M)[cols]
+    got = D.float().cpu().numpy().reshape(N, M)[cols]
+    bad, worst = elem_ok(got, ref)
+    assert bad == 0, worst
