@@ -352,10 +352,12 @@ def run_extras(a, torch, np, tl, C, dev):
         B = torch.randn(S * S, generator=g2).to(dev)
         D = torch.empty(S * S, device=dev)
         res = {}
-        for comp, name in ((C.COMPUTE_DEFAULT, "f32_mfma"), (C.COMPUTE_BF16X3, "bf16x3")):
+        for comp, name in ((C.COMPUTE_DEFAULT, "f32_mfma"), (C.COMPUTE_BF16X3, "bf16x3"),
+                           (C.COMPUTE_BF16X3, "bf16x3_regsplit")):
             p = tl.make_problem("NN", C.F32, S, S, S, S, S, S, S, compute=comp)
-            # the bf16x3 planes go to a workspace allocated once, as a caller would keep it
-            nws = tl.workspace_size(p)
+            # the bf16x3 planes go to a workspace allocated once, as a caller would keep it;
+            # the register-split form (no workspace) splits inside the GEMM
+            nws = 0 if name == "bf16x3_regsplit" else tl.workspace_size(p)
             ws = torch.empty(nws, dtype=torch.uint8, device=dev) if nws else None
             # a few untimed launches first: each form runs at its own clock under the power limit
             ms = time_launches(lambda: tl.run(p, A, B, None, D, workspace=ws), 5, 3, torch, min_ms=300)
@@ -369,6 +371,9 @@ def run_extras(a, torch, np, tl, C, dev):
             torch.cuda.synchronize()
             err = (torch.linalg.norm(D[: S * cols].double() - Dref) / torch.linalg.norm(Dref)).item()
             res[name] = {"tflops": round(tf, 1), "ms": round(ms, 3), "rel_fro_err_vs_fp64": err}
+            if name == "bf16x3_regsplit":
+                t = tl.query_tile(p)
+                res[name]["tile"] = [t.mt_m, t.mt_n]
         res["bf16x3_over_f32"] = round(res["bf16x3"]["tflops"] / res["f32_mfma"]["tflops"], 2)
         ex["precision_8192"] = res
         del A, B, D
