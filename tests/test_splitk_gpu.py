@@ -198,3 +198,36 @@ def test_tail_split(cuda, name, shape, beta):
     got = D.float().cpu().numpy().reshape(N, M)[cols]
     bad, worst = elem_ok(got, ref)
     assert bad == 0, worst
+
+
+def test_tail_split_graph_replay(cuda):
+    """A tail-split plan (the data-parallel launch, the split slices and the
+    reduction) captured into a hipGraph replays to the eager result, with
+    eager work between replays."""
+    M = N = K = 4608
+    p = tl.make_problem("TN", C.BF16, M, N, K, K, K, M, M)
+    n = tl.workspace_size(p)
+    assert n > 0
+    g = torch.Generator(device="cpu").manual_seed(7)
+    A = (torch.randn(M * K, generator=g) / np.sqrt(K)).to(torch.bfloat16).to(cuda)
+    B = (torch.randn(N * K, generator=g) / np.sqrt(K)).to(torch.bfloat16).to(cuda)
+    ws = torch.empty(n, dtype=torch.uint8, device=cuda)
+    D0 = torch.empty(M * N, dtype=torch.bfloat16, device=cuda)
+    tl.run(p, A, B, None, D0, workspace=ws)
+    D = torch.zeros(M * N, dtype=torch.bfloat16, device=cuda)
+    side = torch.cuda.Stream()
+    side.wait_stream(torch.cuda.current_stream())
+    with torch.cuda.stream(side):
+        tl.run(p, A, B, None, D, workspace=ws, stream=side)
+    torch.cuda.current_stream().wait_stream(side)
+    torch.cuda.synchronize()
+    graph = torch.cuda.CUDAGraph()
+    with torch.cuda.graph(graph):
+        tl.run(p, A, B, None, D, workspace=ws)
+    for i in range(3):
+        D.zero_()
+        torch.cuda.synchronize()
+        graph.replay()
+        torch.cuda.synchronize()
+        assert torch.equal(D, D0), i
+        (A.float() * 2).sum().item()   # eager work between replays
