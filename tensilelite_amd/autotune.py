@@ -211,7 +211,8 @@ TUNE_PROBLEMS = (
     + [(C.BF16, "TN", M, N, K, None) for (M, N, K) in
        ((2048, 2048, 8192), (8192, 1024, 8192), (1024, 8192, 8192), (16384, 16384, 1024), (3072, 3072, 3072),
         (6144, 6144, 6144), (2048, 8192, 2048), (8192, 2048, 2048), (1024, 1024, 16384), (12288, 4096, 4096))]
-    + [(C.BF16, "NN", M, N, K, None) for (M, N, K) in ((2048, 4096, 4096), (8192, 8192, 2048), (3072, 3072, 3072))])
+    + [(C.BF16, "NN", M, N, K, None) for (M, N, K) in ((2048, 4096, 4096), (8192, 8192, 2048), (3072, 3072, 3072))]
+    + [(dt, "NT", 8192, 8192, 8192, None) for dt in (C.BF16, C.F16)])
 
 
 def _operands(dt, lay, M, N, K, out, dev):

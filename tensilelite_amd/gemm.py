@@ -65,8 +65,8 @@ def make_tile(tile) -> C.Tile | None:
 
     waves_m x waves_n picks the wave tiling: 2 x 4 the eight-wave engine,
     2 x 2 a four-wave one, 0 x 0 (or absent) the tile's default.  mfma_mn 16
-    picks 16 x 16 x 32 MFMAs (bf16 / fp16 at 64 x 64 and 128 x 128); 0 or 32
-    the 32 x 32 ones.  split_k >= 2 cuts K into that many slices (with a
+    picks 16 x 16 MFMAs at 64 x 64 and 128 x 128 (16 x 16 x 32 for bf16 /
+    fp16, 16 x 16 x 4 for fp32); 0 or 32 the 32 x 32 ones.  split_k >= 2 cuts K into that many slices (with a
     workspace only)."""
     if tile is None:
         return None
