@@ -17,15 +17,20 @@
 //
 // LDS (160 KiB): 128 bytes of K per stage and row, with XOR-swizzled 16-byte
 // chunks (conflict-free fragment reads), B in three stages and A in two.
-// Every load goes HBM -> LDS directly (buffer_load_dwordx4 ... lds) two
-// K-tiles ahead:
-//   * group g loads half g (128 rows) of B for K-tile kt+2 at the start of
-//     its memory window of K-tile kt (B stage (kt+2) % 3 was last read in
-//     K-tile kt-1), and
-//   * its own half g of A for K-tile kt+2 at the start of its compute window
-//     of K-tile kt (only group g reads that half, and it has just finished
-//     reading K-tile kt from it).
-// So every load has two to four windows to land before it is read.
+// Every load goes HBM -> LDS directly (buffer_load_dwordx4 ... lds).  With
+// both operands K-contiguous (TN, fp8) only memory windows issue loads:
+// issuing a load costs its wave tens of cycles, which a compute window would
+// take from the MFMA pipe.  Each memory window issues four loads per wave, two
+// before the fragment reads of each K-step:
+//   * the first memory window of K-tile kt: the group's half g (128 rows) of
+//     A for K-tile kt+1 (only group g reads that half; stage (kt+1) % 2 was
+//     last read in K-tile kt-1), landing four windows later;
+//   * the second: half g of B for K-tile kt+2 (stage (kt+2) % 3 was last read
+//     by both groups in K-tile kt-1), landing five windows later.
+// So the load traffic is spread evenly over the four windows of a K-tile.
+// NT keeps B(kt+2) at the start of the first memory window and A(kt+2) at the
+// start of the second compute window (its memory windows already carry 24
+// transposing reads).
 #pragma once
 #include "gemm_tile_v2.hpp"
 
@@ -78,10 +83,12 @@ struct HalfLoader {
         K = K_;
         lds_off = (128 * h + 32 * s) * RB;
     }
+    // loads q = Q0 .. Q0 + NQ - 1 of K-tile kt (rows and K past the matrix read zero)
+    template <int Q0 = 0, int NQ = NLD>
     __device__ inline void issue(int kt, char* stage) const {
         char* dst = stage + lds_off;
 #pragma unroll
-        for (int q = 0; q < NLD; ++q) {
+        for (int q = Q0; q < Q0 + NQ; ++q) {
             const int c = x ^ q;
             const bool ok = (QROWS * q < rlim) && (kt * DU + c * CE < K);
             const uint32_t off = ok ? base + q * qstride + (uint32_t)kt * RB + (uint32_t)c * 16 : kOOB;
@@ -98,11 +105,12 @@ struct HalfLoader {
     // K-tile's byte offset goes in the instruction's scalar offset, so the loads
     // need no per-lane arithmetic.  Rows past the matrix read zero through
     // their kOOB lane offset (out of range with or without the scalar part).
+    template <int Q0 = 0, int NQ = NLD>
     __device__ inline void issue_full(int kt, char* stage, const uint32_t (&v)[NLD]) const {
         char* dst = stage + lds_off;
         const int soff = __builtin_amdgcn_readfirstlane(kt * RB);
 #pragma unroll
-        for (int q = 0; q < NLD; ++q)
+        for (int q = Q0; q < Q0 + NQ; ++q)
             __builtin_amdgcn_raw_ptr_buffer_load_lds(rsrc, (lds_ptr_t)(dst + q * 1024), 16, v[q], soff, 0, 0);
     }
 };
@@ -135,10 +143,11 @@ struct MNHalfLoader {
         kstride = (uint32_t)(64 * ld * 2);
         lds_off = 16 * s * 256;
     }
+    template <int Q0 = 0, int NQ = NLD>
     __device__ inline void issue(int kt, char* region) const {
         char* dst = region + lds_off;
 #pragma unroll
-        for (int q = 0; q < NLD; ++q) {
+        for (int q = Q0; q < Q0 + NQ; ++q) {
             const bool ok = kt * 64 + 4 * q < klim;
             const uint32_t off = ok ? base + (uint32_t)kt * kstride + q * qstride : kOOB;
             __builtin_amdgcn_raw_ptr_buffer_load_lds(rsrc, (lds_ptr_t)(dst + q * 1024), 16, off, 0, 0, 0);
@@ -147,7 +156,10 @@ struct MNHalfLoader {
     // (no scalar-offset form: rows past the matrix are masked only by the
     // range check of the per-lane offset)
     __device__ inline void full_offsets(uint32_t (&)[NLD]) const {}
-    __device__ inline void issue_full(int kt, char* region, const uint32_t (&)[NLD]) const { issue(kt, region); }
+    template <int Q0 = 0, int NQ = NLD>
+    __device__ inline void issue_full(int kt, char* region, const uint32_t (&)[NLD]) const {
+        issue<Q0, NQ>(kt, region);
+    }
 };
 
 typedef short i16x4_t __attribute__((ext_vector_type(4)));
@@ -257,12 +269,26 @@ __device__ inline void gemm_tile(const TileArgs& a, int tm, int tn, int b, char*
 
     const int KT = (a.K + DU - 1) / DU;
 
-    // prologue: K-tiles 0 and 1 in flight; K-tile 0 complete before the barrier
+    // Load schedule.  BAL (both operands K-contiguous: TN, fp8): loads only in
+    // memory windows, four per wave each (see the top of this file).  With
+    // transposing reads (NT) a memory window already carries 24 LDS reads; its
+    // loads then measured faster where they were: B(kt+2) at the start of the
+    // first memory window and A(kt+2) at the start of the second compute window
+    // (NT bf16 4096^3: 1029 against 633 TFLOP/s with the balanced schedule).
+    constexpr bool BAL = AK && BK;
+
+    // prologue.  BAL: B(0), A(0) and B(1) in flight, B(0) and A(0) complete
+    // before the barrier (A(1) is issued by the first memory window).
+    // Otherwise: K-tiles 0 and 1 in flight, K-tile 0 complete.
     lb.issue(0, stB(0) + lhalf_b);
     la.issue(0, stA(0) + lhalf_a);
     lb.issue(1, stB(1) + lhalf_b);
-    la.issue(1, stA(1) + lhalf_a);
-    wait_vmcnt<2 * NLD>();
+    if constexpr (BAL) {
+        wait_vmcnt<NLD>();
+    } else {
+        la.issue(1, stA(1) + lhalf_a);
+        wait_vmcnt<2 * NLD>();
+    }
     lds_barrier();
     if (grp == 1) barrier();  // group 1 runs one window behind group 0
 
@@ -274,14 +300,21 @@ __device__ inline void gemm_tile(const TileArgs& a, int tm, int tn, int b, char*
     if constexpr (AK) la.full_offsets(va);
     if constexpr (BK) lb.full_offsets(vb);
     const int KF = a.K / DU;
-    auto loadB = [&](int kt, auto full) {
-        if constexpr (BK && decltype(full)::value) lb.issue_full(kt, stB(kt) + lhalf_b, vb);
-        else lb.issue(kt, stB(kt) + lhalf_b);
+    // loads q = Q0 .. Q0 + NQ - 1 of a K-tile of this group's half
+    auto loadB = [&](int kt, auto full, auto q0, auto nq) {
+        constexpr int Q0 = decltype(q0)::value, NQ = decltype(nq)::value;
+        if constexpr (BK && decltype(full)::value) lb.template issue_full<Q0, NQ>(kt, stB(kt) + lhalf_b, vb);
+        else lb.template issue<Q0, NQ>(kt, stB(kt) + lhalf_b);
     };
-    auto loadA = [&](int kt, char* stage, auto full) {
-        if constexpr (AK && decltype(full)::value) la.issue_full(kt, stage + lhalf_a, va);
-        else la.issue(kt, stage + lhalf_a);
+    auto loadA = [&](int kt, char* stage, auto full, auto q0, auto nq) {
+        constexpr int Q0 = decltype(q0)::value, NQ = decltype(nq)::value;
+        if constexpr (AK && decltype(full)::value) la.template issue_full<Q0, NQ>(kt, stage + lhalf_a, va);
+        else la.template issue<Q0, NQ>(kt, stage + lhalf_a);
     };
+    using q0_t = std::integral_constant<int, 0>;
+    using q2_t = std::integral_constant<int, 2>;
+    using n2_t = std::integral_constant<int, 2>;
+    using n4_t = std::integral_constant<int, NLD>;
 
     // Windows per K-tile: memory(half 0), compute(half 0), memory(half 1),
     // compute(half 1); a half is K = 32 (two MFMA K-steps).
@@ -290,29 +323,45 @@ __device__ inline void gemm_tile(const TileArgs& a, int tm, int tn, int b, char*
         const char* sb = stB(kt) + boff;
         const char* ra = stA(kt) + aregion;
         const char* rb = stB(kt) + bregion;
+        auto frags = [&](int half, int k2) {
+            const int ks = 2 * half + k2;
+#pragma unroll
+            for (int i = 0; i < TI; ++i) {
+                if constexpr (AK) fa[k2][i] = *(const i32x4_t*)(sa + P[ks ^ (2 * (i & 1))] + 4096 * i);
+                else fa[k2][i] = tr_frag(ra, TA[i], ks);
+            }
+#pragma unroll
+            for (int j = 0; j < TJ; ++j) {
+                if constexpr (BK) fb[k2][j] = *(const i32x4_t*)(sb + P[ks ^ (2 * j)] + 4096 * j);
+                else fb[k2][j] = tr_frag(rb, TB[j], ks);
+            }
+        };
 #pragma unroll
         for (int half = 0; half < 2; ++half) {
             // ---- memory window
-            if (half == 0) loadB(kt + 2, full);
-#pragma unroll
-            for (int k2 = 0; k2 < 2; ++k2) {
-                const int ks = 2 * half + k2;
-#pragma unroll
-                for (int i = 0; i < TI; ++i) {
-                    if constexpr (AK) fa[k2][i] = *(const i32x4_t*)(sa + P[ks ^ (2 * (i & 1))] + 4096 * i);
-                    else fa[k2][i] = tr_frag(ra, TA[i], ks);
-                }
-#pragma unroll
-                for (int j = 0; j < TJ; ++j) {
-                    if constexpr (BK) fb[k2][j] = *(const i32x4_t*)(sb + P[ks ^ (2 * j)] + 4096 * j);
-                    else fb[k2][j] = tr_frag(rb, TB[j], ks);
-                }
+            if constexpr (BAL) {
+                // each K-step's fragment reads behind two loads, of A(kt+1)
+                // (half 0) or B(kt+2) (half 1)
+                if (half == 0) loadA(kt + 1, stA(kt + 1), full, q0_t{}, n2_t{});
+                else loadB(kt + 2, full, q0_t{}, n2_t{});
+                frags(half, 0);
+                if (half == 0) loadA(kt + 1, stA(kt + 1), full, q2_t{}, n2_t{});
+                else loadB(kt + 2, full, q2_t{}, n2_t{});
+                frags(half, 1);
+            } else {
+                if (half == 0) loadB(kt + 2, full, q0_t{}, n4_t{});
+                frags(half, 0);
+                frags(half, 1);
             }
+            // BAL: B(kt+1), issued by this wave two memory windows ago, lands
+            // before the barrier that precedes its first reader
             if (half == 1) wait_vmcnt<2 * NLD>();
             lds_barrier();
             __builtin_amdgcn_sched_barrier(0);
             // ---- compute window
-            if (half == 1) loadA(kt + 2, stA(kt), full);  // this group's half of A is consumed
+            if constexpr (!BAL) {  // this group's half of A(kt) is consumed
+                if (half == 1) loadA(kt + 2, stA(kt), full, q0_t{}, n4_t{});
+            }
             __builtin_amdgcn_s_setprio(1);
             if constexpr (MODE == kF8) {
 #pragma unroll
@@ -329,11 +378,16 @@ __device__ inline void gemm_tile(const TileArgs& a, int tm, int tn, int b, char*
             }
             __builtin_amdgcn_s_setprio(0);
             __builtin_amdgcn_sched_barrier(0);
-            if (half == 1) wait_vmcnt<2 * NLD>();
+            // BAL: A(kt+1) of this group lands before its first reader's
+            // window; otherwise K-tile kt+1 lands before the next window
+            if (half == 1) {
+                if constexpr (BAL) wait_vmcnt<NLD>();
+                else wait_vmcnt<2 * NLD>();
+            }
             barrier();
         }
     };
-    // K-tiles whose loads (of K-tile kt+2) lie wholly inside K, then the rest
+    // K-tiles whose loads lie wholly inside K, then the rest
     int kt = 0;
     for (; kt < KF - 2; ++kt) ktile(kt, std::true_type{});
     for (; kt < KT; ++kt) ktile(kt, std::false_type{});
