@@ -34,7 +34,7 @@ def _args():
     ap.add_argument("--warmup", type=int, default=200)
     ap.add_argument("--no-extras", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--only", default="", help="run only this extra (profiling)")
+    ap.add_argument("--only", default="", help="run only these extras, comma-separated (profiling)")
     ap.add_argument("--preload-s", type=float, default=1.5,
                     help="untimed seconds of the headline kernel before its warm-up (clock settling)")
     return ap.parse_args()
@@ -263,10 +263,14 @@ def run_extras(a, torch, np, tl, C, dev):
     M = N = K = 4096
     flop = 2.0 * M * N * K
     g = torch.Generator(device="cpu").manual_seed(1)
-    only = a.only
+    only = set(x for x in a.only.split(",") if x)
 
     def want(name):
-        return not only or only == name
+        if only and name not in only:
+            return False
+        # progress on stderr (stdout carries only the JSON line)
+        print(f"[bench] {name}", file=sys.stderr, flush=True)
+        return True
 
     # fp16 / bf16 across layouts and beta.  The forms are timed in three
     # interleaved rounds and each reports its median block, so no form pays

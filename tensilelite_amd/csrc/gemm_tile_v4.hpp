@@ -269,6 +269,15 @@ __device__ inline void gemm_tile(const TileArgs& a, int tm, int tn, int b, char*
 
     const int KT = (a.K + DU - 1) / DU;
 
+    // fp8: the per-tensor quantisation scales, read before the K loop so their
+    // latency is not added to the epilogue
+    float alpha = a.alpha;
+    if constexpr (MODE == kF8) {
+        const float sa = a.scaleA ? *a.scaleA : 1.0f;
+        const float sb = a.scaleB ? *a.scaleB : 1.0f;
+        alpha = alpha / (sa * sb);
+    }
+
     // Load schedule.  BAL (both operands K-contiguous: TN, fp8): loads only in
     // memory windows, four per wave each (see the top of this file).  With
     // transposing reads (NT) a memory window already carries 24 LDS reads; its
@@ -395,12 +404,6 @@ __device__ inline void gemm_tile(const TileArgs& a, int tm, int tn, int b, char*
     // before the workgroup's LDS is released
     wait_vmcnt<0>();
 
-    float alpha = a.alpha;
-    if constexpr (MODE == kF8) {  // per-tensor quantisation scales
-        const float sa = a.scaleA ? *a.scaleA : 1.0f;
-        const float sb = a.scaleB ? *a.scaleB : 1.0f;
-        alpha = alpha / (sa * sb);
-    }
     const char* Cb = (const char*)a.C + (int64_t)b * a.bsC * ((OUT == kOutF32) ? 4 : 2);
     char* Db = (char*)a.D + (int64_t)b * a.bsD * ((OUT == kOutF32) ? 4 : 2);
     int fm[TI * TJ], fn[TI * TJ];
