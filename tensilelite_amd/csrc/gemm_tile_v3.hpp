@@ -83,6 +83,9 @@ struct RegLoader {
 template <int NM, int NR, int SYNC, int MASK>
 __device__ inline void sched_block(int nx) { v2::sched_block<NM, NR, SYNC, MASK>(nx); }
 
+template <int NM, int NR, int NX, int SYNC>
+__device__ inline void sched_mix() { v2::sched_mix<NM, NR, NX, SYNC>(); }
+
 // CPRE: preload C in the epilogue.  The launcher takes it only with beta != 0
 // (the beta = 0 kernel keeps the register allocation it has without it).
 template <int MODE, bool AK, bool BK, int OUT, bool CPRE = true>
@@ -147,14 +150,14 @@ __device__ inline void gemm_tile(const TileArgs& a, int tm, int tn, int b, char*
         constexpr int S = decltype(slot)::value;  // kt % 2
         const char* sa = stA(kt);
         const char* sb = stB(kt);
-        // B's direct loads of K-tile kt were issued two iterations ago; only
-        // the NV loads of the last iteration may still be in flight
-        if constexpr (BK) wait_vmcnt<NV>();
+        // B's direct loads of K-tile kt were issued two iterations ago: the
+        // loads of the last iteration and the A loads issued after them may
+        // still be in flight
+        if constexpr (BK) wait_vmcnt<NV + LA::NVMEM>();
         lds_barrier();  // K-tile kt complete in LDS; K-tile kt-1 fully consumed
-        // K-tile kt+2: A into register slot S (its K-tile kt is in LDS now),
-        // B into stage (kt+2)%3 (K-tile kt-1 is done)
-        la.template issue<S>(kt + 2, nullptr);
-        lb.template issue<S>(kt + 2, stB(kt + 2));
+        // K-tile kt+2: B into stage (kt+2)%3 (K-tile kt-1 is done) during the
+        // first read step, A into register slot S (its K-tile kt is in LDS
+        // now) during the second
         static_for<RS>([&](auto step) {
             constexpr int R = decltype(step)::value;
             constexpr int CUR = R & 1, PREV = (R + 1) & 1;
@@ -175,7 +178,11 @@ __device__ inline void gemm_tile(const TileArgs& a, int tm, int tn, int b, char*
                 lb.template store<S ^ 1>(stB(kt + 1));
                 sched_block<T * T, 2 * T, R + 1, 0x200>(NW);
             } else if constexpr (R == 0) {
-                sched_block<T * T, 2 * T, R + 1, 0x020>(NV);
+                lb.template issue<S>(kt + 2, stB(kt + 2));
+                sched_mix<T * T, 2 * T, LB::NVMEM, R + 1>();
+            } else if constexpr (R == 1) {
+                la.template issue<S>(kt + 2, nullptr);
+                sched_mix<T * T, 2 * T, LA::NVMEM, R + 1>();
             } else {
                 sched_block<T * T, 2 * T, R + 1, 0x020>(0);
             }
@@ -325,10 +332,8 @@ __device__ inline void gemm_tiles_stream(TileAt&& tile_at, int begin, int end, c
         constexpr int S = decltype(slot)::value;
         const char* sa = sA + S * STAGE;
         const char* sb = stB(s);
-        if constexpr (BK) wait_vmcnt<NV>();
+        if constexpr (BK) wait_vmcnt<NV + LA::NVMEM>();
         lds_barrier();
-        la.template issue<S>(kl, nullptr);
-        lb.template issue<S>(kl, stB(s + 2));
         static_for<RS>([&](auto rstep) {
             constexpr int R = decltype(rstep)::value;
             constexpr int CUR = R & 1, PREV = (R + 1) & 1;
@@ -349,7 +354,11 @@ __device__ inline void gemm_tiles_stream(TileAt&& tile_at, int begin, int end, c
                 lb.template store<S ^ 1>(stB(s + 1));
                 sched_block<T * T, 2 * T, R + 1, 0x200>(NW);
             } else if constexpr (R == 0) {
-                sched_block<T * T, 2 * T, R + 1, 0x020>(NV);
+                lb.template issue<S>(kl, stB(s + 2));
+                sched_mix<T * T, 2 * T, LB::NVMEM, R + 1>();
+            } else if constexpr (R == 1) {
+                la.template issue<S>(kl, nullptr);
+                sched_mix<T * T, 2 * T, LA::NVMEM, R + 1>();
             } else {
                 sched_block<T * T, 2 * T, R + 1, 0x020>(0);
             }
