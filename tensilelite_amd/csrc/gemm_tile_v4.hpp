@@ -28,9 +28,10 @@
 //   * the second: half g of B for K-tile kt+2 (stage (kt+2) % 3 was last read
 //     by both groups in K-tile kt-1), landing five windows later.
 // So the load traffic is spread evenly over the four windows of a K-tile.
-// NT keeps B(kt+2) at the start of the first memory window and A(kt+2) at the
-// start of the second compute window (its memory windows already carry 24
-// transposing reads).
+// NT keeps B(kt+2) at the start of the first memory window and A(kt+2) in the
+// second compute window, one load before each group of four MFMAs (its memory
+// windows already carry 24 transposing reads; splitting B's loads around them
+// measured 15-25 % slower).
 #pragma once
 #include "gemm_tile_v2.hpp"
 
@@ -368,15 +369,29 @@ __device__ inline void gemm_tile(const TileArgs& a, int tm, int tn, int b, char*
             lds_barrier();
             __builtin_amdgcn_sched_barrier(0);
             // ---- compute window
-            if constexpr (!BAL) {  // this group's half of A(kt) is consumed
-                if (half == 1) loadA(kt + 2, stA(kt), full, q0_t{}, n4_t{});
-            }
             __builtin_amdgcn_s_setprio(1);
             if constexpr (MODE == kF8) {
 #pragma unroll
                 for (int i = 0; i < TI; ++i)
 #pragma unroll
                     for (int j = 0; j < TJ; ++j) mfma_f8(fa[0][i], fa[1][i], fb[0][j], fb[1][j], acc[i][j]);
+            } else if constexpr (!BAL) {
+                // this group's half of A(kt) is consumed: A(kt+2)'s four loads
+                // go one before each group of four MFMAs (half 1)
+#pragma unroll
+                for (int k2 = 0; k2 < 2; ++k2)
+#pragma unroll
+                    for (int i = 0; i < TI; ++i) {
+                        if (half == 1 && (i & 1) == 0) {
+                            if (k2 == 0 && i == 0) loadA(kt + 2, stA(kt), full, std::integral_constant<int, 0>{}, std::integral_constant<int, 1>{});
+                            if (k2 == 0 && i == 2) loadA(kt + 2, stA(kt), full, std::integral_constant<int, 1>{}, std::integral_constant<int, 1>{});
+                            if (k2 == 1 && i == 0) loadA(kt + 2, stA(kt), full, std::integral_constant<int, 2>{}, std::integral_constant<int, 1>{});
+                            if (k2 == 1 && i == 2) loadA(kt + 2, stA(kt), full, std::integral_constant<int, 3>{}, std::integral_constant<int, 1>{});
+                        }
+#pragma unroll
+                        for (int j = 0; j < TJ; ++j) v2::mfma16<MODE>(fa[k2][i], fb[k2][j], acc[i][j]);
+                        if ((i & 1) == 1) __builtin_amdgcn_sched_barrier(0);
+                    }
             } else {
 #pragma unroll
                 for (int k2 = 0; k2 < 2; ++k2)
