@@ -86,6 +86,70 @@ __device__ inline void sched_block(int nx) { v2::sched_block<NM, NR, SYNC, MASK>
 template <int NM, int NR, int NX, int SYNC>
 __device__ inline void sched_mix() { v2::sched_mix<NM, NR, NX, SYNC>(); }
 
+// Epilogue of a whole tile with beta = 0 (after the K loop, LDS free): the
+// four waves' 128 x 128 blocks go to LDS as a column-major tile (column stride
+// 528 bytes, free of bank conflicts), then every instruction stores whole
+// 512-byte column pieces (8 columns per pass of 256 threads; fp32 D in two
+// rounds of 128 rows), non-temporal.  The per-fragment stores write 16-byte
+// pieces of 32 columns per instruction.  Dt: the tile's first element.
+template <int OUT>
+__device__ inline void store_tile_staged(const TileArgs& a, char* Dt, const f32x16_t (&acc)[4][4], char* smem,
+                                         int tid) {
+    constexpr int OS = (OUT == kOutF32) ? 4 : 2;
+    constexpr int CS = 528;           // LDS bytes per staged column (512 + 16)
+    constexpr int ROWS = 512 / OS;    // rows per staging round: 256 (16-bit), 128 (fp32)
+    constexpr int NPG = MT / ROWS;    // staging rounds: 1 or 2
+    constexpr int NT = 2;             // buffer store: non-temporal
+    const int lane = tid & 63;
+    const int wave = tid >> 6;
+    const int wm = wave & 1, wn = wave >> 1;
+    const int g = lane >> 5;
+    const float alpha = a.alpha;
+    const bool trunc = OUT == kOutBF16 && (a.flags & kFlagTrunc) != 0;
+    const __amdgpu_buffer_rsrc_t rd = make_rsrc(Dt, (uint32_t)((int64_t)(MT - 1) * a.ldd * OS + MT * OS));
+#pragma unroll
+    for (int pg = 0; pg < NPG; ++pg) {
+        if (NPG == 1 || wm == pg) {
+#pragma unroll
+            for (int i = 0; i < 4; ++i)
+#pragma unroll
+                for (int j = 0; j < 4; ++j) {
+                    const f32x16_t& x = acc[i][j];
+                    const int ml = (NPG == 1 ? 128 * wm : 0) + 32 * i + 16 * g;
+                    const int nl = 128 * wn + 32 * j + (lane & 31);
+                    char* p = smem + nl * CS + ml * OS;
+                    if constexpr (OUT == kOutF32) {
+#pragma unroll
+                        for (int q = 0; q < 4; ++q)
+                            *(f32x4_t*)(p + 16 * q) = f32x4_t{alpha * x[4 * q], alpha * x[4 * q + 1],
+                                                              alpha * x[4 * q + 2], alpha * x[4 * q + 3]};
+                    } else {
+#pragma unroll
+                        for (int q = 0; q < 2; ++q) {
+                            i32x4_t v;
+#pragma unroll
+                            for (int e = 0; e < 4; ++e) {
+                                const float lo = alpha * x[8 * q + 2 * e], hi = alpha * x[8 * q + 2 * e + 1];
+                                v[e] = (int)(OUT == kOutF16 ? pack_f16x2(lo, hi)
+                                                            : (trunc ? pack_bf16x2_trunc(lo, hi) : pack_bf16x2_rne(lo, hi)));
+                            }
+                            *(i32x4_t*)(p + 16 * q) = v;
+                        }
+                    }
+                }
+        }
+        lds_barrier();
+#pragma unroll
+        for (int pass = 0; pass < 32; ++pass) {
+            const int col = 8 * pass + (tid >> 5), ch = tid & 31;
+            const i32x4_t v = *(const i32x4_t*)(smem + col * CS + ch * 16);
+            const uint32_t off = (uint32_t)(((int64_t)col * a.ldd + ROWS * pg) * OS + ch * 16);
+            __builtin_amdgcn_raw_buffer_store_b128(v, rd, off, 0, NT);
+        }
+        if (pg + 1 < NPG) lds_barrier();
+    }
+}
+
 // CPRE: preload C in the epilogue.  The launcher takes it only with beta != 0
 // (the beta = 0 kernel keeps the register allocation it has without it).
 template <int MODE, bool AK, bool BK, int OUT, bool CPRE = true>
@@ -205,6 +269,15 @@ __device__ inline void gemm_tile(const TileArgs& a, int tm, int tn, int b, char*
 
     const char* Cb = (const char*)a.C + (int64_t)b * a.bsC * ((OUT == kOutF32) ? 4 : 2);
     char* Db = (char*)a.D + (int64_t)b * a.bsD * ((OUT == kOutF32) ? 4 : 2);
+    // a whole tile with beta = 0 leaves through LDS as whole columns
+    if constexpr (!CPRE) {
+        constexpr int OS = (OUT == kOutF32) ? 4 : 2;
+        if (m0 + MT <= a.M && n0 + MT <= a.N && (a.flags & kFlagBeta) == 0 &&
+            (int64_t)(MT - 1) * a.ldd * OS + MT * OS < 0x7ffffff0ll) {
+            store_tile_staged<OUT>(a, Db + ((int64_t)n0 * a.ldd + m0) * OS, acc, smem, tid);
+            return;
+        }
+    }
     // 16-bit C with beta != 0: every fragment's C slice is requested before the
     // first is used (the loader and fragment registers are free by now); a
     // fragment not wholly inside D reads C's first rows instead and stores
