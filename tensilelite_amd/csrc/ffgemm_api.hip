@@ -464,7 +464,8 @@ int prepare_with(const ffgemm_problem* p, const void* A, const void* B, const vo
     a.scaleA = p->scaleA; a.scaleB = p->scaleB;
     a.tiles_m = (int32_t)tiles_m; a.tiles_n = (int32_t)tiles_n; a.batch = (int32_t)batch;
     a.group_m = group_m > 0 ? group_m : default_group_m(tiles_m, tiles_n);
-    a.flags = (use_c ? kFlagBeta : 0u) | (p->round == FFGEMM_ROUND_TRUNCATE ? kFlagTrunc : 0u);
+    a.flags = (use_c ? kFlagBeta : 0u) | (p->round == FFGEMM_ROUND_TRUNCATE ? kFlagTrunc : 0u) |
+              (grid <= cu_count() ? kFlagOneRound : 0u);
     L.mode = mode; L.out = out; L.layout = p->layout; L.tile = t; L.grid = grid; L.waves = waves; L.mfma = mfma;
     L.any = any;
     return FFGEMM_SUCCESS;

@@ -122,5 +122,6 @@ struct TileArgs {
 };
 constexpr uint32_t kFlagBeta = 1u;   // beta != 0: C is read
 constexpr uint32_t kFlagTrunc = 2u;  // bf16 output by truncation
+constexpr uint32_t kFlagOneRound = 4u;  // every tile's workgroup is resident at once (grid <= CUs)
 
 }  // namespace ffgemm

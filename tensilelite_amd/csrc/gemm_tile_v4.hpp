@@ -205,9 +205,12 @@ __device__ inline void mfma_f8(const i32x4_t& a0, const i32x4_t& a1, const i32x4
 // 512-byte column pieces: 16 passes of 16 columns (bf16 / fp16), or two rounds
 // of 128 rows for fp32 D (the 256 KiB tile exceeds LDS).  The per-fragment
 // stores write 16-byte pieces of 32 columns per instruction, so every 128-byte
-// line of D was assembled from several partial writes; whole-line stores,
-// non-temporal, measured fp8 4096^3 63.3 -> 55.3 us, 8192^3 412.8 -> 367.7 us,
-// bf16 4096^3 103.1 -> 100.9 us.  Dt: the tile's first element.
+// line of D was assembled from several partial writes; whole-line stores
+// measured fp8 4096^3 63.3 -> 55.3 us, 8192^3 412.8 -> 367.7 us, bf16 4096^3
+// 103.1 -> 100.9 us.  Stores are non-temporal, or system-coherent (sc0 sc1)
+// for fp32 D when all tiles run in one round: fp8 4096^3 then measured 2.5-4 %
+// faster (and 2 % slower at 8192^3, four rounds; bf16 D 0.4 % slower).  Dt:
+// the tile's first element.
 template <int OUT>
 __device__ inline void store_tile_staged(const TileArgs& a, char* Dt, const f32x16_t (&acc)[TI][TJ], float alpha,
                                          char* smem, int tid) {
@@ -215,12 +218,13 @@ __device__ inline void store_tile_staged(const TileArgs& a, char* Dt, const f32x
     constexpr int CS = 528;              // LDS bytes per staged column (512 + 16)
     constexpr int ROWS = 512 / OS;       // rows per staging round: 256 (16-bit), 128 (fp32)
     constexpr int NPG = MT / ROWS;       // staging rounds: 1 or 2
-    constexpr int NT = 2;                // buffer store: non-temporal
+    constexpr int NT = 2, SC01 = 17;     // buffer store cache policies: nt, sc0 sc1
     const int lane = tid & 63;
     const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
     const int grp = wave >> 2, s = wave & 3;
     const int h = lane >> 5;
     const bool trunc = OUT == kOutBF16 && (a.flags & kFlagTrunc) != 0;
+    const bool one_round = OUT == kOutF32 && (a.flags & kFlagOneRound) != 0;
     const __amdgpu_buffer_rsrc_t rd = make_rsrc(Dt, (uint32_t)((int64_t)(MT - 1) * a.ldd * OS + MT * OS));
     // the groups realign (group 1 ran one barrier ahead), then every wave's
     // loads have landed and the stages are free
@@ -263,7 +267,8 @@ __device__ inline void store_tile_staged(const TileArgs& a, char* Dt, const f32x
             const int col = 16 * pass + (tid >> 5), ch = tid & 31;
             const i32x4_t v = *(const i32x4_t*)(smem + col * CS + ch * 16);
             const uint32_t off = (uint32_t)(((int64_t)col * a.ldd + ROWS * pg) * OS + ch * 16);
-            __builtin_amdgcn_raw_buffer_store_b128(v, rd, off, 0, NT);
+            if (one_round) __builtin_amdgcn_raw_buffer_store_b128(v, rd, off, 0, SC01);
+            else __builtin_amdgcn_raw_buffer_store_b128(v, rd, off, 0, NT);
         }
         if (pg + 1 < NPG) lds_barrier();
     }
