@@ -300,6 +300,42 @@ def run_extras(a, torch, np, tl, C, dev):
         ex[name] = {"tflops": round(tf, 1), "ms": round(ms, 4), "pct_peak": round(100 * tf / 2500.0, 1)}
     del bufs
 
+    # bf16 at 8192^3 per layout: the library's own plan (ffgemm_run_ws; NN, NT
+    # and TT repack their M/N-contiguous operands K-contiguous and run as TN),
+    # the native engine of the layout (repack never, no workspace) and torch's
+    # matmul (hipBLASLt) on the same column-major buffers, for context; three
+    # interleaved rounds, best of each
+    if want("layouts_8192"):
+        S8 = 8192
+        fl8 = 2.0 * S8 ** 3
+        X8 = (torch.randn(S8 * S8, generator=g) / math.sqrt(S8)).to(torch.bfloat16).to(dev)
+        Y8 = (torch.randn(S8 * S8, generator=g) / math.sqrt(S8)).to(torch.bfloat16).to(dev)
+        D8 = torch.empty(S8 * S8, dtype=torch.bfloat16, device=dev)
+        runs8, plans8, wss = {}, {}, []
+        for layout in ("NN", "NT", "TT", "TN"):
+            p8 = tl.make_problem(layout, C.BF16, S8, S8, S8, S8, S8, S8, S8)
+            nws = tl.workspace_size(p8)
+            ws8 = torch.empty(nws, dtype=torch.uint8, device=dev) if nws else None
+            wss.append(ws8)
+            plans8[layout] = "repack" if tl.query_tile_ws(p8).repack == 2 else "native"
+            # D^T (N x M row-major) = op(B)^T op(A)^T over the same buffers
+            at = X8.view(S8, S8) if layout[0] == "N" else X8.view(S8, S8).t()
+            bt = Y8.view(S8, S8) if layout[1] == "N" else Y8.view(S8, S8).t()
+            runs8[layout] = {
+                "library": (lambda p8=p8, ws8=ws8: tl.run(p8, X8, Y8, None, D8, workspace=ws8)),
+                "native": (lambda p8=p8: tl.run(p8, X8, Y8, None, D8, tile=(0, 0, 0, 0, 0, 0, 0, 1), workspace=None)),
+                "hipblaslt": (lambda at=at, bt=bt: torch.matmul(bt, at, out=D8.view(S8, S8))),
+            }
+        best8 = {(l, k): float("inf") for l in runs8 for k in runs8[l]}
+        for _ in range(3):
+            for l in runs8:
+                for k, fn in runs8[l].items():
+                    best8[(l, k)] = min(best8[(l, k)], time_launches(fn, 5, 2, torch, min_ms=100))
+        ex["bf16_layouts_8192"] = {
+            l: {"plan": plans8[l], **{f"{k}_tflops": round(fl8 / (best8[(l, k)] * 1e-3) / 1e12, 1) for k in runs8[l]},
+                "library_ms": round(best8[(l, "library")], 4)} for l in runs8}
+        del X8, Y8, D8, wss, runs8
+
     # torch (hipBLASLt) bf16 at the same size, for context only: the headline
     # problem itself (A and B K-contiguous: D^T = B A^T in torch's row-major
     # terms) and a plain row-major torch.mm, in interleaved rounds with this

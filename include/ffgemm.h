@@ -100,6 +100,13 @@ typedef struct ffgemm_tile {
                        * need the fp32 partial planes in a workspace, so only
                        * ffgemm_run_ws splits (single problems, not bf16x3);
                        * ffgemm_run takes 0 or 1. */
+    int32_t repack;   /* K-major repack (bf16/fp16, layouts NN, NT, TT): 0 = the
+                       * library's choice, 1 = never, 2 = always.  An operand
+                       * stored M/N-contiguous is copied K-contiguous into the
+                       * workspace and the problem runs as TN, on the engine
+                       * the K-contiguous layout has; only ffgemm_run_ws
+                       * repacks (it needs K a multiple of 8, 16-byte aligned
+                       * strides and pointers, and batch <= 65535). */
 } ffgemm_tile;
 
 /* Library / device bring-up.  Optional: every entry point initialises lazily. */
@@ -175,19 +182,32 @@ int ffgemm_run(const ffgemm_problem *p, const void *A, const void *B, const void
  *    pass runs as two sub-problems, the whole passes as one launch and the
  *    remaining tile columns (or rows) with split-K; the workspace holds the
  *    remainder's planes.  ffgemm_query_tile_ws then still reports the
- *    single-launch record. */
+ *    single-launch record;
+ *  - K-major repack: a bf16/fp16 problem in layout NN, NT or TT copies its
+ *    M/N-contiguous operand(s) K-contiguous (rows x K per batch) into the
+ *    front of the workspace and runs as TN, through this same path, with the
+ *    rest of the workspace for the TN problem's own plan.  Taken with no tile
+ *    given (and no tuning-table entry) when the library's cost model puts it
+ *    ahead, or when a tile record asks for it (repack 2).  No counterpart in
+ *    the reference, whose only layout is TN. */
 int ffgemm_workspace_size(const ffgemm_problem *p, size_t *bytes);
 
+/* ffgemm_workspace_size for the plan ffgemm_run_ws follows with this tile
+ * record (NULL: the library's choice, as ffgemm_workspace_size); a record
+ * with repack 2 or split_k >= 2 and mt_m 0 keeps the library's tile. */
+int ffgemm_workspace_size_ex(const ffgemm_problem *p, const ffgemm_tile *tile, size_t *bytes);
+
 /* The tile record ffgemm_run_ws follows for this problem given a workspace of
- * ffgemm_workspace_size bytes, split_k included (ffgemm_query_tile reports
- * the ffgemm_run plan, which never splits).  *out as input: a tile to
- * complete, or zeros. */
+ * ffgemm_workspace_size bytes, split_k and repack included (ffgemm_query_tile
+ * reports the ffgemm_run plan, which never splits or repacks; a repacking
+ * plan reports the TN problem's record with repack 2).  *out as input: a tile
+ * to complete, or zeros. */
 int ffgemm_query_tile_ws(const ffgemm_problem *p, ffgemm_tile *out);
 
 /* ffgemm_run with a caller-owned device workspace (256-byte aligned).  Falls
  * back to ffgemm_run when the workspace is NULL or smaller than
  * ffgemm_workspace_size reports (RTN_ERR instead when the tile record asks
- * for split_k >= 2).  Calls on one workspace must not overlap. */
+ * for split_k >= 2 or repack 2).  Calls on one workspace must not overlap. */
 int ffgemm_run_ws(const ffgemm_problem *p, const void *A, const void *B, const void *C, void *D,
                   const ffgemm_tile *tile, void *workspace, size_t workspace_bytes, void *stream);
 

@@ -38,6 +38,7 @@ class Tile(ctypes.Structure):
         ("mt_m", ctypes.c_int32), ("mt_n", ctypes.c_int32), ("depth_u", ctypes.c_int32),
         ("waves_m", ctypes.c_int32), ("waves_n", ctypes.c_int32), ("mfma_mn", ctypes.c_int32),
         ("group_m", ctypes.c_int32), ("lds_buffers", ctypes.c_int32), ("split_k", ctypes.c_int32),
+        ("repack", ctypes.c_int32),
     ]
 
     def as_tuple(self):
@@ -70,6 +71,8 @@ def lib() -> ctypes.CDLL:
     L.ffgemm_run.argtypes = [PP, vp, vp, vp, vp, PT, vp]; L.ffgemm_run.restype = i32
     L.ffgemm_workspace_size.argtypes = [PP, ctypes.POINTER(ctypes.c_size_t)]
     L.ffgemm_workspace_size.restype = i32
+    L.ffgemm_workspace_size_ex.argtypes = [PP, PT, ctypes.POINTER(ctypes.c_size_t)]
+    L.ffgemm_workspace_size_ex.restype = i32
     L.ffgemm_run_ws.argtypes = [PP, vp, vp, vp, vp, PT, vp, ctypes.c_size_t, vp]; L.ffgemm_run_ws.restype = i32
     L.ffgemm_run_x3_noalloc.argtypes = [PP, vp, vp, vp, vp, PT, vp]; L.ffgemm_run_x3_noalloc.restype = i32
     u32 = ctypes.c_uint32

@@ -152,6 +152,57 @@ __global__ void __launch_bounds__(256) x3_split_kernel(const float* __restrict__
     }
 }
 
+// Each lane moves one 8 x 8 block of (r, k): eight 16-byte loads along r (one
+// per k), a transpose in registers, eight 16-byte stores along k (one per r).
+// Lane (a, c) = (lane & 7, lane >> 3) takes rows 8a..8a+7 and k 8c..8c+7 of
+// its wave's 64 x 64 block, so every load instruction reads eight whole
+// 128-byte runs of x (one per k) and every store writes eight of y (one per
+// r).  A workgroup's four waves take 64 rows and 256 consecutive k; the
+// linear workgroup index runs along r first, so neighbouring workgroups read
+// neighbouring runs of the same rows of x.  x is read once (non-temporal
+// loads); y is stored normally, since the GEMM reads it next.
+__global__ void __launch_bounds__(256) kpack16_kernel(const uint16_t* __restrict__ x, int64_t rows, int64_t K,
+                                                      int64_t ldx, int64_t bsx, uint16_t* __restrict__ y,
+                                                      int64_t ldy, int64_t bsy) {
+    typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    const int64_t tiles_r = (rows + 63) / 64;
+    const int64_t tr = blockIdx.x % tiles_r, tk = blockIdx.x / tiles_r;
+    const int64_t r0 = tr * 64 + 8 * (lane & 7);
+    const int64_t k0 = (tk * 4 + w) * 64 + 8 * (lane >> 3);
+    if (r0 >= rows || k0 >= K) return;  // no barriers below; K % 8 == 0, so k0 + 7 < K
+    const uint16_t* xb = x + (int64_t)blockIdx.y * bsx + k0 * ldx + r0;
+    uint16_t* yb = y + (int64_t)blockIdx.y * bsy + r0 * ldy + k0;
+    const int nr = rows - r0 < 8 ? (int)(rows - r0) : 8;  // rows of this block inside the operand
+    u32x4 in[8];
+    if (nr == 8) {
+#pragma unroll
+        for (int i = 0; i < 8; ++i) in[i] = __builtin_nontemporal_load((const u32x4*)(xb + i * ldx));
+    } else {
+        // a short last block: element loads, nothing past the operand's last row
+#pragma unroll
+        for (int i = 0; i < 8; ++i) {
+            uint16_t e[8];
+#pragma unroll
+            for (int j = 0; j < 8; ++j) e[j] = j < nr ? xb[i * ldx + j] : (uint16_t)0;
+#pragma unroll
+            for (int q = 0; q < 4; ++q) in[i][q] = (uint32_t)e[2 * q] | ((uint32_t)e[2 * q + 1] << 16);
+        }
+    }
+    // out[j] holds k 0..7 of row j: element k of row j is element j of in[k]
+#pragma unroll
+    for (int j = 0; j < 8; ++j) {
+        if (j >= nr) break;
+        u32x4 o;
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            const uint32_t lo = in[2 * q][j >> 1], hi = in[2 * q + 1][j >> 1];
+            o[q] = (j & 1) ? ((lo >> 16) | (hi & 0xffff0000u)) : ((lo & 0xffffu) | (hi << 16));
+        }
+        *(u32x4*)(yb + j * ldy) = o;
+    }
+}
+
 __global__ void convert_kernel(const float* __restrict__ src, void* __restrict__ dst, int64_t n, int to) {
     for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
         float v = src[i];

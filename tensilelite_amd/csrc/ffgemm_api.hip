@@ -924,37 +924,173 @@ int run_tail(const ffgemm_problem* p, const void* A, const void* B, const void* 
     if (r != FFGEMM_SUCCESS) return r;
     return run_split(&q1, A1, B1, C1, D1, LT, ws, s);
 }
+
+// K-major repack (workspace path, bf16/fp16).  The engines run fastest with
+// both operands K-contiguous (TN); at 256 x 256 the other layouts reach
+// 0.84-0.86 (NN), 0.76-0.84 (TT) and 0.71-0.78 (NT) of its rate on MI355X
+// (bf16, 4096^3 and 8192^3), their M/N-contiguous operands going through
+// register transposes or transposing LDS reads.  Such a problem can instead
+// copy those operands K-contiguous into the workspace (kpack16_kernel,
+// HBM-bound) and run as TN, through this same workspace path.  The library
+// takes it when the model below puts it 5 % ahead: at 8192^3 the measured
+// gain is 10 % (NN), 22 % (TT) and 23 % (NT); at 4096^3 the copy costs about
+// what the layout loses, and the model declines.
+struct RepackPlan {
+    bool a, b;        // repack A (stored M-contiguous), B (stored N-contiguous)
+    int64_t bytes_a;  // workspace bytes of A's copy (256-byte rounded); B's follows
+    int64_t bytes_b;
+};
+
+// Copy rate of kpack16_kernel for the model: bytes read plus written per µs
+// (4.0-5.0 TB/s measured, the launch included).
+constexpr double kRepackBytesPerUs = 4.5e6;
+
+// The problem the repacked operands form: TN, packed K-contiguous copies
+// (leading dimension K, batch stride rows * K).
+ffgemm_problem repacked_problem(const ffgemm_problem* p, const RepackPlan& rp) {
+    ffgemm_problem q = *p;
+    q.layout = FFGEMM_TN;
+    if (rp.a) { q.lda = p->K; q.bsA = p->M * p->K; }
+    if (rp.b) { q.ldb = p->K; q.bsB = p->N * p->K; }
+    return q;
+}
+
+// Whether p (native plan L) repacks under `tile` (its repack field: 0 the
+// model's choice, 1 never, 2 always).  A forced repack the problem cannot take
+// is an error; TN, K = 0 and empty problems have nothing to repack.
+int plan_repack(const ffgemm_problem* p, const ffgemm_tile* tile, const Launch& L, RepackPlan& rp, bool& take) {
+    take = false;
+    const int force = tile ? tile->repack : 0;
+    if (force < 0 || force > 2) return fail(FFGEMM_ERR, "repack %d (0, 1 or 2)", force);
+    if (force == 1 || p->layout == FFGEMM_TN || L.grid <= 0 || p->K <= 0) return FFGEMM_SUCCESS;
+    const int64_t batch = p->batch > 0 ? p->batch : 1;
+    const char* need = (L.mode != kBF16 && L.mode != kF16) ? "bf16/fp16 operands"
+                       : L.any ? "16-byte aligned leading dimensions, batch strides and pointers"
+                       : (p->K % 8) ? "K a multiple of 8"
+                       : batch > 65535 ? "batch <= 65535"
+                       : nullptr;
+    if (need) return force == 2 ? fail(FFGEMM_ERR, "repack needs %s", need) : FFGEMM_SUCCESS;
+    const bool ta = p->layout >= FFGEMM_TN, tb = (p->layout & 1) != 0;
+    rp.a = !ta;
+    rp.b = tb;
+    rp.bytes_a = rp.a ? round256(p->M * p->K * 2 * batch) : 0;
+    rp.bytes_b = rp.b ? round256(p->N * p->K * 2 * batch) : 0;
+    if (force == 2) {
+        take = true;
+        return FFGEMM_SUCCESS;
+    }
+    // the library's own choice: no caller tile, no tuning-table entry, a
+    // single 256 x 256 launch (split-K plans keep their layout)
+    ffgemm_tile tuned;
+    if ((tile && tile->mt_m) || L.split > 1 || L.tile.m != 256 || L.tile.n != 256 || L.waves || L.mfma ||
+        tuned_tile(p, &tuned))
+        return FFGEMM_SUCCESS;
+    // model (µs): passes of 256 x 256 tiles at 70 % of the MFMA rate at 1.9 GHz
+    // plus 4 µs each, over the layout's relative rate; the copies at
+    // kRepackBytesPerUs plus 3 µs per launch
+    static const double lay_eff[4] = {0.85, 0.75, 1.0, 0.82};  // NN, NT, TN, TT
+    const double passes = (double)cdiv(cdiv(p->M, 256) * cdiv(p->N, 256) * batch, cu_count());
+    const double tn = passes * (2.0 * 256 * 256 * (double)p->K / (4096.0 * 1.9e3 * 0.7) + 4.0);
+    const double moved = 2.0 * (double)(rp.bytes_a + rp.bytes_b);
+    const double cost = tn + moved / kRepackBytesPerUs + 3.0 * ((rp.a ? 1 : 0) + (rp.b ? 1 : 0));
+    take = cost < 0.95 * tn / lay_eff[p->layout];
+    return FFGEMM_SUCCESS;
+}
+
+// Runs a repack plan: the copies into the front of the workspace, then the TN
+// problem on the workspace path with the rest of it.
+int run_repack(const ffgemm_problem* p, const void* A, const void* B, const void* C, void* D,
+               const ffgemm_tile* tile, const RepackPlan& rp, void* ws, size_t ws_bytes, hipStream_t s) {
+    const int64_t batch = p->batch > 0 ? p->batch : 1;
+    uint16_t* wa = (uint16_t*)ws;
+    uint16_t* wb = (uint16_t*)((char*)ws + rp.bytes_a);
+    if (rp.a)
+        hipLaunchKernelGGL(kpack16_kernel, dim3((unsigned)(cdiv(p->M, 64) * cdiv(p->K, 256)), (unsigned)batch),
+                           dim3(256), 0, s, (const uint16_t*)A, p->M, p->K, p->lda, p->bsA, wa, p->K, p->M * p->K);
+    if (rp.b)
+        hipLaunchKernelGGL(kpack16_kernel, dim3((unsigned)(cdiv(p->N, 64) * cdiv(p->K, 256)), (unsigned)batch),
+                           dim3(256), 0, s, (const uint16_t*)B, p->N, p->K, p->ldb, p->bsB, wb, p->K, p->N * p->K);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return hip_fail(e, "repack launch");
+    const ffgemm_problem q = repacked_problem(p, rp);
+    ffgemm_tile t;
+    if (tile) t = *tile;
+    else memset(&t, 0, sizeof(t));
+    t.repack = 1;
+    const size_t used = (size_t)(rp.bytes_a + rp.bytes_b);
+    void* rest = ws_bytes > used ? (void*)((char*)ws + used) : nullptr;
+    return ffgemm_run_ws(&q, rp.a ? (const void*)wa : A, rp.b ? (const void*)wb : B, C, D, &t, rest,
+                         rest ? ws_bytes - used : 0, s);
+}
 }  // namespace
 
 extern "C" {
 
-int ffgemm_workspace_size(const ffgemm_problem* p, size_t* bytes) {
+int ffgemm_workspace_size_ex(const ffgemm_problem* p, const ffgemm_tile* tile, size_t* bytes) {
     if (!p || !bytes) return fail(FFGEMM_ERR, "null pointer");
     *bytes = 0;
     if (x3_planes_apply(p)) {
         *bytes = (size_t)x3_ws_bytes(p);
         return FFGEMM_SUCCESS;
     }
-    // the split-K partial planes of the plan ffgemm_run_ws would follow
-    // (validation with dummy aligned pointers; nothing is dereferenced)
+    // the repacked operands, then the split-K partial planes of the plan
+    // ffgemm_run_ws would follow (validation with dummy aligned pointers;
+    // nothing is dereferenced)
     static const int64_t dummy[4] = {0, 0, 0, 0};
     Launch L;
-    if (p->dtype != FFGEMM_F64 && split_plan(p, dummy, dummy, dummy, (void*)dummy, nullptr, L) == FFGEMM_SUCCESS) {
+    if (p->dtype != FFGEMM_F64 && split_plan(p, dummy, dummy, dummy, (void*)dummy, tile, L) == FFGEMM_SUCCESS) {
+        RepackPlan rp;
+        bool take = false;
+        const int r = plan_repack(p, tile, L, rp, take);
+        if (r != FFGEMM_SUCCESS) return r;
+        if (take) {
+            const ffgemm_problem q = repacked_problem(p, rp);
+            ffgemm_tile t;
+            if (tile) t = *tile;
+            else memset(&t, 0, sizeof(t));
+            t.repack = 1;
+            size_t rest = 0;
+            const int rq = ffgemm_workspace_size_ex(&q, &t, &rest);
+            if (rq != FFGEMM_SUCCESS) return rq;
+            *bytes = (size_t)(rp.bytes_a + rp.bytes_b) + rest;
+            return FFGEMM_SUCCESS;
+        }
         TailPlan tp;
         Launch LT;
         if (L.split > 1) *bytes = (size_t)split_ws_bytes(L);
-        else if (tail_plan_for(p, nullptr, L, tp, LT)) *bytes = (size_t)split_ws_bytes(LT);  // the remainder's planes
+        else if (tail_plan_for(p, tile, L, tp, LT)) *bytes = (size_t)split_ws_bytes(LT);  // the remainder's planes
     }
     return FFGEMM_SUCCESS;
 }
+
+int ffgemm_workspace_size(const ffgemm_problem* p, size_t* bytes) { return ffgemm_workspace_size_ex(p, nullptr, bytes); }
 
 int ffgemm_query_tile_ws(const ffgemm_problem* p, ffgemm_tile* out) {
     if (!p || !out) return fail(FFGEMM_ERR, "null pointer");
     if (x3_planes_apply(p) || p->dtype == FFGEMM_F64) return ffgemm_query_tile(p, out);
     static const int64_t dummy[4] = {0, 0, 0, 0};
     Launch L;
-    const int r = split_plan(p, dummy, dummy, dummy, (void*)dummy, out->mt_m ? out : nullptr, L);
+    const ffgemm_tile* given = (out->mt_m || out->repack || out->split_k) ? out : nullptr;
+    const int r = split_plan(p, dummy, dummy, dummy, (void*)dummy, given, L);
     if (r != FFGEMM_SUCCESS) return r;
+    {
+        // a repacking plan reports the TN problem's record, with repack 2
+        RepackPlan rp;
+        bool take = false;
+        const int rr = plan_repack(p, given, L, rp, take);
+        if (rr != FFGEMM_SUCCESS) return rr;
+        if (take) {
+            const ffgemm_problem q = repacked_problem(p, rp);
+            ffgemm_tile t = *out;
+            if (!given) memset(&t, 0, sizeof(t));
+            t.repack = 1;
+            const int rq = ffgemm_query_tile_ws(&q, &t);
+            if (rq != FFGEMM_SUCCESS) return rq;
+            *out = t;
+            out->repack = 2;
+            return FFGEMM_SUCCESS;
+        }
+    }
     if (L.split <= 1) return ffgemm_query_tile(p, out);
     const EngineGeometry geo = launch_geometry(L);
     memset(out, 0, sizeof(*out));
@@ -972,6 +1108,22 @@ int ffgemm_run_ws(const ffgemm_problem* p, const void* A, const void* B, const v
         Launch L;
         const int r = split_plan(p, A, B, C, D, tile, L);
         if (r != FFGEMM_SUCCESS) return r;
+        RepackPlan rp;
+        bool take = false;
+        const int rr = plan_repack(p, tile, L, rp, take);
+        if (rr != FFGEMM_SUCCESS) return rr;
+        if (take) {
+            size_t need = 0;
+            const int rs = ffgemm_workspace_size_ex(p, tile, &need);
+            if (rs != FFGEMM_SUCCESS) return rs;
+            if (workspace && workspace_bytes >= need) {
+                if ((uintptr_t)workspace % 256) return fail(FFGEMM_ERR, "workspace must be 256-byte aligned");
+                return run_repack(p, A, B, C, D, tile, rp, workspace, workspace_bytes, (hipStream_t)stream);
+            }
+            if (tile && tile->repack == 2)
+                return fail(FFGEMM_ERR, "repack needs a workspace of %zu bytes (%zu given)", need,
+                            workspace ? workspace_bytes : (size_t)0);
+        }
         if (L.split > 1) {
             const int64_t need = split_ws_bytes(L);
             if (workspace && workspace_bytes >= (size_t)need) {

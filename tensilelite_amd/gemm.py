@@ -60,14 +60,15 @@ def make_problem(layout, dtype, M, N, K, lda, ldb, ldc, ldd, alpha=1.0, beta=0.0
 
 
 def make_tile(tile) -> C.Tile | None:
-    """Tile record from (mt_m, mt_n[, group_m[, waves_m, waves_n[, mfma_mn[, split_k]]]])
+    """Tile record from (mt_m, mt_n[, group_m[, waves_m, waves_n[, mfma_mn[, split_k[, repack]]]]])
     or a Tile.
 
     waves_m x waves_n picks the wave tiling: 2 x 4 the eight-wave engine,
     2 x 2 a four-wave one, 0 x 0 (or absent) the tile's default.  mfma_mn 16
     picks 16 x 16 MFMAs at 64 x 64 and 128 x 128 (16 x 16 x 32 for bf16 /
     fp16, 16 x 16 x 4 for fp32); 0 or 32 the 32 x 32 ones.  split_k >= 2 cuts K into that many slices (with a
-    workspace only)."""
+    workspace only).  repack 2 copies M/N-contiguous 16-bit operands K-contiguous into the workspace and runs
+    the problem as TN, 1 never does, 0 leaves it to the library (mt_m 0 keeps the library's tile)."""
     if tile is None:
         return None
     if isinstance(tile, C.Tile):
@@ -82,13 +83,18 @@ def make_tile(tile) -> C.Tile | None:
         t.mfma_mn = int(tile[5])
     if len(tile) > 6:
         t.split_k = int(tile[6])
+    if len(tile) > 7:
+        t.repack = int(tile[7])
     return t
 
 
-def workspace_size(problem: C.Problem) -> int:
-    """Device bytes ffgemm_run_ws can use for this problem (0: no workspace path)."""
+def workspace_size(problem: C.Problem, tile=None) -> int:
+    """Device bytes ffgemm_run_ws can use for this problem under `tile` (None:
+    the library's choice); 0: no workspace path."""
     n = ctypes.c_size_t()
-    C.check(C.lib().ffgemm_workspace_size(ctypes.byref(problem), ctypes.byref(n)))
+    t = make_tile(tile)
+    C.check(C.lib().ffgemm_workspace_size_ex(ctypes.byref(problem), ctypes.byref(t) if t is not None else None,
+                                             ctypes.byref(n)))
     return n.value
 
 
@@ -112,7 +118,7 @@ def run(problem: C.Problem, A, B, Cm, D, tile=None, stream=None, workspace="auto
     t = make_tile(tile)
     tp = ctypes.byref(t) if t is not None else None
     if isinstance(workspace, str):
-        n = workspace_size(problem)
+        n = workspace_size(problem, t)
         workspace = torch.empty(n, dtype=torch.uint8, device=torch.device("cuda", torch.cuda.current_device())) if n else None
     if workspace is None:
         C.check(C.lib().ffgemm_run(ctypes.byref(problem), _ptr(A), _ptr(B), _ptr(Cm), _ptr(D), tp, _stream(stream)))

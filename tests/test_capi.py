@@ -570,3 +570,71 @@ def test_split_k_rejections(case):
     if case == "run":
         st = C.lib().ffgemm_run(ctypes.byref(p), fake, fake, None, fake, ctypes.byref(t), None)
         assert st == C.ERR and "workspace" in C.lib().ffgemm_last_error().decode()
+
+
+def test_repack_planning():
+    """The workspace path repacks a bf16/fp16 NN/NT/TT problem's M/N-contiguous
+    operands K-contiguous (rows x K bf16 per operand at the front of the
+    workspace) and runs it as TN when the cost model puts that ahead: large
+    problems (8192^3), not 4096^3, never TN; the no-workspace plan
+    (ffgemm_query_tile) never repacks; a caller's tile, or repack 1, keeps the
+    native layout; repack 2 forces it at any size."""
+    cus = C.lib().ffgemm_device_cu_count()
+    if cus != 256:
+        pytest.skip(f"planning figures assume 256 CUs ({cus} here)")
+    S = 8192
+    for layout, nbytes in [("NN", S * S * 2), ("TT", S * S * 2), ("NT", 2 * S * S * 2), ("TN", 0)]:
+        p = _p(layout, C.BF16, S, S, S)
+        assert tl.workspace_size(p) == nbytes, layout
+        assert tl.query_tile_ws(p).repack == (2 if nbytes else 0), layout
+        assert tl.query_tile(p).repack == 0
+        assert tl.workspace_size(p, (256, 256)) == 0
+        assert tl.workspace_size(p, (0, 0, 0, 0, 0, 0, 0, 1)) == 0
+    for layout in ("NN", "NT", "TT"):
+        assert tl.workspace_size(_p(layout, C.BF16, 4096, 4096, 4096)) == 0, layout
+    # forced: each repacked operand's bytes rounded to 256, then the TN problem's own plan
+    p = _p("NT", C.F16, 1000, 520, 256)
+    t = tl.query_tile_ws(p, tl.make_tile((0, 0, 0, 0, 0, 0, 0, 2)))
+    tn = tl.query_tile_ws(_p("TN", C.F16, 1000, 520, 256))
+    assert t.repack == 2 and (t.mt_m, t.mt_n, t.waves_m, t.waves_n) == (tn.mt_m, tn.mt_n, tn.waves_m, tn.waves_n)
+    assert tl.workspace_size(p, (0, 0, 0, 0, 0, 0, 0, 2)) == (1000 * 256 * 2 + 255) // 256 * 256 + 520 * 256 * 2
+    # a forced repack whose TN problem splits: the partial planes follow the copies
+    p = _p("NN", C.BF16, 520, 480, 1024)
+    rec = (128, 128, 0, 0, 0, 0, 2, 2)
+    t = tl.query_tile_ws(p, tl.make_tile(rec))
+    assert (t.mt_m, t.mt_n, t.split_k, t.repack) == (128, 128, 2, 2)
+    assert tl.workspace_size(p, rec) == 520 * 1024 * 2 + 2 * 520 * 480 * 4
+
+
+@pytest.mark.parametrize("case", ["f32", "x3", "k_odd", "off_grid", "value", "no_ws"])
+def test_repack_rejections(case):
+    """A forced repack the problem cannot take is refused with RTN_ERR, before
+    anything runs: fp32 and bf16x3 operands, K not a multiple of 8, strides off
+    the 16-byte grid, a repack value outside 0-2, and ffgemm_run_ws without
+    the workspace it needs."""
+    fake = ctypes.c_void_p(0x100000)
+    rec = (0, 0, 0, 0, 0, 0, 0, 3 if case == "value" else 2)
+    if case == "f32":
+        p = _p("NN", C.F32, 512, 512, 512)
+    elif case == "x3":
+        p = _p("NN", C.F32, 512, 512, 512, compute=C.COMPUTE_BF16X3)
+    elif case == "k_odd":
+        p = _p("NT", C.BF16, 512, 512, 516)
+    elif case == "off_grid":
+        p = _p("NN", C.BF16, 512, 512, 512, lda=515)
+    else:
+        p = _p("NN", C.BF16, 512, 512, 512)
+    if case == "x3":
+        # bf16x3 planes take the workspace path of their own: the record's
+        # repack is not consulted there, and its plain run refuses nothing
+        assert tl.workspace_size(p, rec) > 0
+        return
+    if case == "no_ws":
+        t = tl.make_tile(rec)
+        st = C.lib().ffgemm_run_ws(ctypes.byref(p), fake, fake, None, fake, ctypes.byref(t), fake, 256, None)
+        assert st == C.ERR and "workspace" in C.lib().ffgemm_last_error().decode()
+        return
+    with pytest.raises(C.FFGemmError, match="RTN_ERR"):
+        tl.workspace_size(p, rec)
+    with pytest.raises(C.FFGemmError, match="RTN_ERR"):
+        tl.query_tile_ws(p, tl.make_tile(rec))
