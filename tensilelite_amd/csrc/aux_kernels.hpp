@@ -26,14 +26,20 @@ __global__ void __launch_bounds__(256) x3_split_kernel(const float* __restrict__
                                                        uint16_t* __restrict__ out, int64_t kp, int order);
 __global__ void convert_kernel(const float* __restrict__ src, void* __restrict__ dst, int64_t n, int to);
 
-// K-major repack of a 16-bit operand stored M/N-contiguous: element (r, k),
-// at x[b * bsx + k * ldx + r], goes to y[b * bsy + r * ldy + k].  Grid:
-// x = cdiv(rows, 64) * cdiv(K, 256) workgroups of 256 threads, y = batch.
-// Needs 16-byte aligned x and y, ldx, bsx, ldy and bsy multiples of 8
-// elements, and K a multiple of 8.
-__global__ void __launch_bounds__(256) kpack16_kernel(const uint16_t* __restrict__ x, int64_t rows, int64_t K,
-                                                      int64_t ldx, int64_t bsx, uint16_t* __restrict__ y,
-                                                      int64_t ldy, int64_t bsy);
+// K-major repack of 16-bit operands stored M/N-contiguous: element (r, k)
+// of batch b, at x[b * bsx + k * ldx + r], goes to y[b * bsy + r * K + k].
+// One launch repacks up to two operands (o1.blocks 0: one), in 64 x 64
+// blocks: blocks = cdiv(rows, 64) * cdiv(K, 64) per batch.  Needs 16-byte
+// aligned x and y, ldx, bsx and bsy multiples of 8 elements, and K a multiple
+// of 8.  Any grid of 256-thread workgroups covers every block.
+struct KPackOperand {
+    const uint16_t* x;
+    uint16_t* y;
+    int64_t rows, ldx, bsx, bsy;
+    uint32_t tiles_r;  // cdiv(rows, 64)
+    uint32_t blocks;   // 64 x 64 blocks per batch
+};
+__global__ void __launch_bounds__(256) kpack16_kernel(KPackOperand o0, KPackOperand o1, int64_t K, uint32_t batch);
 
 // Split-K reduction: D = alpha * sum_s W[s] + beta * C, converted to D's type.
 // W holds the fp32 partial products of S K-slices, slice s at W + s * slice

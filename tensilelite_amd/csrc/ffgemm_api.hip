@@ -932,9 +932,10 @@ int run_tail(const ffgemm_problem* p, const void* A, const void* B, const void* 
 // register transposes or transposing LDS reads.  Such a problem can instead
 // copy those operands K-contiguous into the workspace (kpack16_kernel,
 // HBM-bound) and run as TN, through this same workspace path.  The library
-// takes it when the model below puts it 5 % ahead: at 8192^3 the measured
-// gain is 10 % (NN), 22 % (TT) and 23 % (NT); at 4096^3 the copy costs about
-// what the layout loses, and the model declines.
+// takes it when the model below puts it 3 % ahead.  Measured on MI355X
+// against the native engines (bf16): 4096^3 +1.6-3.5 % (NN), +2.4-7 % (NT),
+// +4.6-4.8 % (TT); 8192^3 NN +7 %, NT +12 %, TT +19 %; the autograd forward
+// shape (NN 4096 x 8192 x 4096) +8 %.
 struct RepackPlan {
     bool a, b;        // repack A (stored M-contiguous), B (stored N-contiguous)
     int64_t bytes_a;  // workspace bytes of A's copy (256-byte rounded); B's follows
@@ -942,8 +943,8 @@ struct RepackPlan {
 };
 
 // Copy rate of kpack16_kernel for the model: bytes read plus written per µs
-// (4.0-5.0 TB/s measured, the launch included).
-constexpr double kRepackBytesPerUs = 4.5e6;
+// (5.7-6.5 TB/s measured on MI355X, 4096^2 and 8192^2 bf16).
+constexpr double kRepackBytesPerUs = 5.5e6;
 
 // The problem the repacked operands form: TN, packed K-contiguous copies
 // (leading dimension K, batch stride rows * K).
@@ -968,7 +969,9 @@ int plan_repack(const ffgemm_problem* p, const ffgemm_tile* tile, const Launch& 
                        : L.any ? "16-byte aligned leading dimensions, batch strides and pointers"
                        : (p->K % 8) ? "K a multiple of 8"
                        : batch > 65535 ? "batch <= 65535"
-                       : nullptr;
+                       : (cdiv(p->M, 64) + cdiv(p->N, 64)) * cdiv(p->K, 64) * batch >= ((int64_t)1 << 31)
+                           ? "fewer than 2^31 blocks of 64 x 64"
+                           : nullptr;
     if (need) return force == 2 ? fail(FFGEMM_ERR, "repack needs %s", need) : FFGEMM_SUCCESS;
     const bool ta = p->layout >= FFGEMM_TN, tb = (p->layout & 1) != 0;
     rp.a = !ta;
@@ -987,13 +990,16 @@ int plan_repack(const ffgemm_problem* p, const ffgemm_tile* tile, const Launch& 
         return FFGEMM_SUCCESS;
     // model (µs): passes of 256 x 256 tiles at 70 % of the MFMA rate at 1.9 GHz
     // plus 4 µs each, over the layout's relative rate; the copies at
-    // kRepackBytesPerUs plus 3 µs per launch
-    static const double lay_eff[4] = {0.85, 0.75, 1.0, 0.82};  // NN, NT, TN, TT
+    // kRepackBytesPerUs plus 2 µs for their launch
+    // the layouts' rates relative to TN (NN, NT, TN, TT): fp16's TN runs at a
+    // lower clock under the power limit, so its other layouts lose less
+    static const double eff_bf16[4] = {0.85, 0.76, 1.0, 0.83}, eff_f16[4] = {0.88, 0.82, 1.0, 0.85};
+    const double* lay_eff = L.mode == kF16 ? eff_f16 : eff_bf16;
     const double passes = (double)cdiv(cdiv(p->M, 256) * cdiv(p->N, 256) * batch, cu_count());
     const double tn = passes * (2.0 * 256 * 256 * (double)p->K / (4096.0 * 1.9e3 * 0.7) + 4.0);
     const double moved = 2.0 * (double)(rp.bytes_a + rp.bytes_b);
-    const double cost = tn + moved / kRepackBytesPerUs + 3.0 * ((rp.a ? 1 : 0) + (rp.b ? 1 : 0));
-    take = cost < 0.95 * tn / lay_eff[p->layout];
+    const double cost = tn + moved / kRepackBytesPerUs + 2.0;  // one copy launch
+    take = cost < 0.97 * tn / lay_eff[p->layout];
     return FFGEMM_SUCCESS;
 }
 
@@ -1004,12 +1010,23 @@ int run_repack(const ffgemm_problem* p, const void* A, const void* B, const void
     const int64_t batch = p->batch > 0 ? p->batch : 1;
     uint16_t* wa = (uint16_t*)ws;
     uint16_t* wb = (uint16_t*)((char*)ws + rp.bytes_a);
-    if (rp.a)
-        hipLaunchKernelGGL(kpack16_kernel, dim3((unsigned)(cdiv(p->M, 64) * cdiv(p->K, 256)), (unsigned)batch),
-                           dim3(256), 0, s, (const uint16_t*)A, p->M, p->K, p->lda, p->bsA, wa, p->K, p->M * p->K);
-    if (rp.b)
-        hipLaunchKernelGGL(kpack16_kernel, dim3((unsigned)(cdiv(p->N, 64) * cdiv(p->K, 256)), (unsigned)batch),
-                           dim3(256), 0, s, (const uint16_t*)B, p->N, p->K, p->ldb, p->bsB, wb, p->K, p->N * p->K);
+    // one launch for both operands: A's blocks (if repacked) first
+    KPackOperand op[2];
+    int n = 0;
+    auto add = [&](const void* x, uint16_t* y, int64_t rows, int64_t ldx, int64_t bsx) {
+        KPackOperand& o = op[n++];
+        o.x = (const uint16_t*)x; o.y = y; o.rows = rows; o.ldx = ldx; o.bsx = bsx; o.bsy = rows * p->K;
+        o.tiles_r = (uint32_t)cdiv(rows, 64);
+        o.blocks = (uint32_t)(cdiv(rows, 64) * cdiv(p->K, 64));
+    };
+    if (rp.a) add(A, wa, p->M, p->lda, p->bsA);
+    if (rp.b) add(B, wb, p->N, p->ldb, p->bsB);
+    if (n == 1) { op[1] = op[0]; op[1].blocks = 0; }
+    const int64_t total = ((int64_t)op[0].blocks + op[1].blocks) * batch;
+    // four waves a workgroup, one 64 x 64 block each per pass; at most four
+    // workgroups a CU, so large operands take several blocks per wave
+    const int64_t grid = std::min<int64_t>(cdiv(total, 4), 4 * (int64_t)cu_count());
+    hipLaunchKernelGGL(kpack16_kernel, dim3((unsigned)grid), dim3(256), 0, s, op[0], op[1], p->K, (uint32_t)batch);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return hip_fail(e, "repack launch");
     const ffgemm_problem q = repacked_problem(p, rp);

@@ -575,10 +575,10 @@ def test_split_k_rejections(case):
 def test_repack_planning():
     """The workspace path repacks a bf16/fp16 NN/NT/TT problem's M/N-contiguous
     operands K-contiguous (rows x K bf16 per operand at the front of the
-    workspace) and runs it as TN when the cost model puts that ahead: large
-    problems (8192^3), not 4096^3, never TN; the no-workspace plan
-    (ffgemm_query_tile) never repacks; a caller's tile, or repack 1, keeps the
-    native layout; repack 2 forces it at any size."""
+    workspace) and runs it as TN when the cost model puts that ahead: problems
+    of whole 256 x 256 passes (4096^3, 8192^3), not small ones, never TN; the
+    no-workspace plan (ffgemm_query_tile) never repacks; a caller's tile, or
+    repack 1, keeps the native layout; repack 2 forces it at any size."""
     cus = C.lib().ffgemm_device_cu_count()
     if cus != 256:
         pytest.skip(f"planning figures assume 256 CUs ({cus} here)")
@@ -591,7 +591,9 @@ def test_repack_planning():
         assert tl.workspace_size(p, (256, 256)) == 0
         assert tl.workspace_size(p, (0, 0, 0, 0, 0, 0, 0, 1)) == 0
     for layout in ("NN", "NT", "TT"):
-        assert tl.workspace_size(_p(layout, C.BF16, 4096, 4096, 4096)) == 0, layout
+        n4 = 4096 * 4096 * 2 * (2 if layout == "NT" else 1)
+        assert tl.workspace_size(_p(layout, C.BF16, 4096, 4096, 4096)) == n4, layout
+        assert tl.workspace_size(_p(layout, C.F16, 1024, 1024, 1024)) == 0, layout
     # forced: each repacked operand's bytes rounded to 256, then the TN problem's own plan
     p = _p("NT", C.F16, 1000, 520, 256)
     t = tl.query_tile_ws(p, tl.make_tile((0, 0, 0, 0, 0, 0, 0, 2)))
