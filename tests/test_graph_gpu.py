@@ -85,6 +85,31 @@ def test_bf16x3_workspace_graph_replays(cuda):
     _replays_match(_capture(fn), [D], [eager])
 
 
+@pytest.mark.parametrize("layout", ["NN", "NT"])
+def test_repack_graph_replays(cuda, layout):
+    """The K-major repack plan (copy launch, then the TN problem on the rest of
+    the workspace, here split in two K slices) captured and replayed."""
+    M, N, K = 520, 392, 1024
+    print(f"case graph repack {layout}", flush=True)
+    g = torch.Generator().manual_seed(9)
+    A = (torch.randn(M * K, generator=g) / K ** 0.5).to(torch.bfloat16).to(cuda)   # M-contiguous
+    B = (torch.randn(K * N, generator=g) / K ** 0.5).to(torch.bfloat16).to(cuda)
+    Cm = torch.randn(M * N, generator=g).to(torch.bfloat16).to(cuda)
+    D = torch.empty(M * N, dtype=torch.bfloat16, device=cuda)
+    p = tl.make_problem(layout, C.BF16, M, N, K, M, K if layout[1] == "N" else N, M, M, 0.5, 1.0)
+    rec = (128, 128, 0, 0, 0, 0, 2, 2)
+    ws = torch.empty(tl.workspace_size(p, rec), dtype=torch.uint8, device=cuda)
+    fn = lambda: tl.run(p, A, B, Cm, D, tile=rec, workspace=ws)
+    fn()
+    torch.cuda.synchronize()
+    eager = D.clone()
+    ref = oracle.gemm_f64(layout, M, N, K, 0.5, A.float().cpu().numpy().astype(np.float64), M,
+                          B.float().cpu().numpy().astype(np.float64), K if layout[1] == "N" else N, 1.0,
+                          Cm.float().cpu().numpy().astype(np.float64), M, M)
+    assert rel_fro(eager.float().cpu().numpy(), ref) < 1e-2
+    _replays_match(_capture(fn), [D], [eager])
+
+
 def test_fp8_quantize_and_gemm_graph_replays(cuda):
     M, N, K = 512, 384, 256
     print("case graph fp8 quantize + gemm", flush=True)
