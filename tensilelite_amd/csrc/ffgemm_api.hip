@@ -933,9 +933,9 @@ int run_tail(const ffgemm_problem* p, const void* A, const void* B, const void* 
 // copy those operands K-contiguous into the workspace (kpack16_kernel,
 // HBM-bound) and run as TN, through this same workspace path.  The library
 // takes it when the model below puts it 3 % ahead.  Measured on MI355X
-// against the native engines (bf16): 4096^3 +1.6-3.5 % (NN), +2.4-7 % (NT),
-// +4.6-4.8 % (TT); 8192^3 NN +7 %, NT +12 %, TT +19 %; the autograd forward
-// shape (NN 4096 x 8192 x 4096) +8 %.
+// against the native engines (bf16): 4096^3 NN +3-4 %, NT +2-7 %, TT +6-7 %;
+// 8192^3 NN +7 %, NT +12 %, TT +19 %; the autograd forward shape (NN 4096 x
+// 8192 x 4096) +11 %.
 struct RepackPlan {
     bool a, b;        // repack A (stored M-contiguous), B (stored N-contiguous)
     int64_t bytes_a;  // workspace bytes of A's copy (256-byte rounded); B's follows
