@@ -983,20 +983,29 @@ int plan_repack(const ffgemm_problem* p, const ffgemm_tile* tile, const Launch& 
         return FFGEMM_SUCCESS;
     }
     // the library's own choice: no caller tile, no tuning-table entry, a
-    // single 256 x 256 launch (split-K plans keep their layout)
+    // single launch of 256 x 256 or 128 x 128 tiles (split-K plans keep their
+    // layout; the repacked problem takes the same tile)
     ffgemm_tile tuned;
-    if ((tile && tile->mt_m) || L.split > 1 || L.tile.m != 256 || L.tile.n != 256 || L.waves || L.mfma ||
-        tuned_tile(p, &tuned))
+    const bool big = L.tile.m == 256 && L.tile.n == 256, mid = L.tile.m == 128 && L.tile.n == 128;
+    if ((tile && tile->mt_m) || L.split > 1 || !(big || mid) || L.waves || L.mfma || tuned_tile(p, &tuned))
         return FFGEMM_SUCCESS;
-    // model (µs): passes of 256 x 256 tiles at 70 % of the MFMA rate at 1.9 GHz
-    // plus 4 µs each, over the layout's relative rate; the copies at
-    // kRepackBytesPerUs plus 2 µs for their launch
-    // the layouts' rates relative to TN (NN, NT, TN, TT): fp16's TN runs at a
-    // lower clock under the power limit, so its other layouts lose less
-    static const double eff_bf16[4] = {0.85, 0.76, 1.0, 0.83}, eff_f16[4] = {0.88, 0.82, 1.0, 0.85};
-    const double* lay_eff = L.mode == kF16 ? eff_f16 : eff_bf16;
-    const double passes = (double)cdiv(cdiv(p->M, 256) * cdiv(p->N, 256) * batch, cu_count());
-    const double tn = passes * (2.0 * 256 * 256 * (double)p->K / (4096.0 * 1.9e3 * 0.7) + 4.0);
+    // model (µs): passes of tiles at 70 % of the MFMA rate at 1.9 GHz times the
+    // tile's relative efficiency (as plan_cost), plus 4 µs each, over the
+    // layout's rate relative to TN; against the same TN time plus the copies
+    // at kRepackBytesPerUs and 2 µs for their launch.  The layouts' relative
+    // rates (NN, NT, TN, TT), measured on MI355X: at 256 x 256, bf16 4096^3
+    // and 8192^3; fp16 at 4096^3, whose TN runs at a lower clock when every CU
+    // is busy, so its other layouts lose less there (a grid that leaves CUs
+    // idle, 3072^3 fp16, lost as much as bf16); at 128 x 128 (2048 x 2048 x
+    // 8192, 1024 x 4096 x 4096), where NT's two register rings cost most.
+    static const double eff_big[4] = {0.85, 0.76, 1.0, 0.83}, eff_big_f16[4] = {0.88, 0.82, 1.0, 0.85};
+    static const double eff_mid[4] = {0.85, 0.60, 1.0, 0.80};
+    const int64_t tiles = cdiv(p->M, L.tile.m) * cdiv(p->N, L.tile.n) * batch;
+    const bool full = tiles >= cu_count();
+    const double* lay_eff = mid ? eff_mid : (L.mode == kF16 && full) ? eff_big_f16 : eff_big;
+    const double passes = (double)cdiv(tiles, cu_count());
+    const double tile_eff = big ? 1.0 : 0.70;
+    const double tn = passes * (2.0 * L.tile.m * L.tile.n * (double)p->K / (4096.0 * 1.9e3 * 0.7 * tile_eff) + 4.0);
     const double moved = 2.0 * (double)(rp.bytes_a + rp.bytes_b);
     const double cost = tn + moved / kRepackBytesPerUs + 2.0;  // one copy launch
     take = cost < 0.97 * tn / lay_eff[p->layout];

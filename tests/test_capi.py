@@ -594,6 +594,14 @@ def test_repack_planning():
         n4 = 4096 * 4096 * 2 * (2 if layout == "NT" else 1)
         assert tl.workspace_size(_p(layout, C.BF16, 4096, 4096, 4096)) == n4, layout
         assert tl.workspace_size(_p(layout, C.F16, 1024, 1024, 1024)) == 0, layout
+        # 256 x 256 tiles that leave CUs idle: every layout repacks, fp16 too
+        assert tl.query_tile_ws(_p(layout, C.F16, 3072, 3072, 3072)).repack == 2, layout
+    # 128 x 128 tiles: NT repacks (its native engine is slowest there); TT at
+    # 1024 x 4096 x 4096 and NN / TT at 2048^3 stay native
+    assert tl.query_tile_ws(_p("NT", C.BF16, 2048, 2048, 8192)).repack == 2
+    assert tl.query_tile_ws(_p("TT", C.BF16, 1024, 4096, 4096)).repack == 0
+    assert tl.query_tile_ws(_p("NN", C.BF16, 2048, 2048, 2048)).repack == 0
+    assert tl.query_tile_ws(_p("TT", C.BF16, 2048, 2048, 2048)).repack == 0
     # forced: each repacked operand's bytes rounded to 256, then the TN problem's own plan
     p = _p("NT", C.F16, 1000, 520, 256)
     t = tl.query_tile_ws(p, tl.make_tile((0, 0, 0, 0, 0, 0, 0, 2)))
