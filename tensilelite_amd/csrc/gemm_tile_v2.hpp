@@ -29,14 +29,18 @@ namespace v2 {
 
 // Per-tile pipeline configuration: 64-byte rows (K = 32 per stage), 3-4
 // stages; two workgroups per CU where the registers allow, so one group's
-// MFMAs cover the other's waits.
-template <int MT_M, int MT_N, bool AK, bool BK>
+// MFMAs cover the other's waits.  64 x 64 with both operands K-contiguous
+// takes 128-byte rows (K = 64 per stage): twice the K in flight and half the
+// barriers (fp16 / bf16 4096 x 512 x 4096 on MI355X 34.9 -> 28.3 us; the
+// one-round shapes of 512 x 512 x 512 to 512 x 2048 x 2048 within 2 %).  RBX
+// > 0 sets the row bytes instead.
+template <int MT_M, int MT_N, bool AK, bool BK, int RBX = 0>
 struct Cfg {
     static constexpr bool kBig = (MT_M == 256 && MT_N == 256);
     static constexpr bool kSmall = (MT_M == 128 && MT_N == 128);
     // (256 x 256 with a K-contiguous B runs the split-ring engine instead;
     // with an M/N-contiguous B, 64-byte rows and four stages measured best)
-    static constexpr int RB = 64;                      // LDS bytes per row per stage
+    static constexpr int RB = RBX ? RBX : ((MT_M == 64 && MT_N == 64 && AK && BK) ? 128 : 64);  // LDS bytes per row per stage
     static constexpr int DU = RB / 2;                  // K per stage (16-bit)
     static constexpr int RS = DU / 16;                 // MFMA read steps per stage
     // two workgroups per CU where the registers allow it without spills: the
@@ -237,10 +241,10 @@ __device__ inline void mfma16(const i32x4_t& a, const i32x4_t& b, f32x16_t& acc)
                                                      acc, 0, 0, 0);
 }
 
-template <int MODE, bool AK, bool BK, int MT_M, int MT_N, int OUT>
+template <int MODE, bool AK, bool BK, int MT_M, int MT_N, int OUT, int RBX = 0>
 __device__ inline void gemm_tile(const TileArgs& a, int tm, int tn, int b, char* smem, int tid) {
     static_assert(MODE == kBF16 || MODE == kF16, "16-bit engine");
-    using C = Cfg<MT_M, MT_N, AK, BK>;
+    using C = Cfg<MT_M, MT_N, AK, BK, RBX>;
     constexpr int RB = C::RB, DU = C::DU, RS = C::RS, NS = C::NS, PF = C::PF;
     constexpr int WT_M = MT_M / 2, WT_N = MT_N / 2;
     constexpr int TM = WT_M / 32, TN = WT_N / 32;
@@ -367,9 +371,9 @@ __device__ inline void gemm_tile(const TileArgs& a, int tm, int tn, int b, char*
 }
 
 // LDS bytes the engine needs for a tile.
-template <int MT_M, int MT_N, bool AK, bool BK>
+template <int MT_M, int MT_N, bool AK, bool BK, int RBX = 0>
 __host__ __device__ constexpr int smem_bytes() {
-    using C = Cfg<MT_M, MT_N, AK, BK>;
+    using C = Cfg<MT_M, MT_N, AK, BK, RBX>;
     return C::NS * (MT_M + MT_N) * C::RB;
 }
 

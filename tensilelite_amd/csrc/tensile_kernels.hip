@@ -100,13 +100,14 @@ using namespace ffgemm;
         FFGEMM_TENSILE_DECODE(MODE, 256, 256)                                                                     \
         v4::gemm_tile<MODE, true, true, OUT>(a, tm, tn, tb, smem, threadIdx.x);                                   \
     }
-// 128 x 128 and 64 x 64 16-bit, DepthU 32: the pipelined four-wave engine.
+// 128 x 128 and 64 x 64 16-bit, DepthU 32 (64-byte K rows, as the names
+// say): the pipelined four-wave engine.
 #define FFGEMM_TENSILE_V2(NAME, MODE, OUT, MT)                                                                    \
-    extern "C" __global__ void __launch_bounds__(kThreads, (v2::Cfg<MT, MT, true, true>::OCC)) NAME(              \
+    extern "C" __global__ void __launch_bounds__(kThreads, (v2::Cfg<MT, MT, true, true, 64>::OCC)) NAME(          \
         FFGEMM_TENSILE_PARAMS) {                                                                                  \
-        __shared__ __attribute__((aligned(16))) char smem[v2::smem_bytes<MT, MT, true, true>()];                  \
+        __shared__ __attribute__((aligned(16))) char smem[v2::smem_bytes<MT, MT, true, true, 64>()];              \
         FFGEMM_TENSILE_DECODE(MODE, MT, MT)                                                                       \
-        v2::gemm_tile<MODE, true, true, MT, MT, OUT>(a, tm, tn, tb, smem, threadIdx.x);                           \
+        v2::gemm_tile<MODE, true, true, MT, MT, OUT, 64>(a, tm, tn, tb, smem, threadIdx.x);                       \
     }
 // fp32, DepthU 32: the register-staged engine.
 #define FFGEMM_TENSILE_V1(NAME, MT)                                                                               \
