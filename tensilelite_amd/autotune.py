@@ -25,7 +25,7 @@ import os
 import torch
 
 from . import _capi as C
-from .gemm import make_problem, make_tile, run
+from .gemm import make_problem, make_tile, query_tile_ws, run
 
 # (mt_m, mt_n, waves_m, waves_n, mfma_mn); waves (0, 0) is the tile's default
 # engine, mfma 0 its 32 x 32 MFMAs
@@ -114,9 +114,14 @@ def tune_gemm(problem: C.Problem, A, B, Cm, D, tiles=TILES, group_ms=(0,), split
                 rec = (t[0], t[1], gm, wm, wn, mf, S)
                 try:
                     run(problem, A, B, Cm, D, make_tile(rec), workspace=ws)
+                    # the split the library runs (slices are at least 128
+                    # elements of K, so a large S can come to fewer slices)
+                    eff = query_tile_ws(problem, make_tile(rec)).split_k if S > 1 else 1
                 except C.FFGemmError:
                     continue  # no kernel for this tile / wave tiling / split with this dtype and layout
-                cands.append(rec)
+                rec = rec[:6] + (max(1, eff),)
+                if rec not in cands:
+                    cands.append(rec)
     best = {}
     for _ in range(rounds):
         for rec in cands:
