@@ -29,18 +29,20 @@ namespace v2 {
 
 // Per-tile pipeline configuration: 64-byte rows (K = 32 per stage), 3-4
 // stages; two workgroups per CU where the registers allow, so one group's
-// MFMAs cover the other's waits.  64 x 64 with both operands K-contiguous
-// takes 128-byte rows (K = 64 per stage): twice the K in flight and half the
-// barriers (fp16 / bf16 4096 x 512 x 4096 on MI355X 34.9 -> 28.3 us; the
-// one-round shapes of 512 x 512 x 512 to 512 x 2048 x 2048 within 2 %).  RBX
-// > 0 sets the row bytes instead.
+// MFMAs cover the other's waits.  64 x 64 and 128 x 128 with both operands
+// K-contiguous take 128-byte rows (K = 64 per stage): twice the K in flight
+// and half the barriers.  Measured on MI355X: 64 x 64 fp16 / bf16 4096 x 512 x
+// 4096 34.9 -> 28.3 us, the one-round shapes of 512 x 512 x 512 to 512 x 2048
+// x 2048 within 2 %; 128 x 128, now one workgroup per CU (128 KiB of LDS),
+// 1-4 % faster from 2048^3 to 4096^3 and on 480 x 8192 x 8192.  RBX > 0 sets
+// the row bytes instead.
 template <int MT_M, int MT_N, bool AK, bool BK, int RBX = 0>
 struct Cfg {
     static constexpr bool kBig = (MT_M == 256 && MT_N == 256);
     static constexpr bool kSmall = (MT_M == 128 && MT_N == 128);
     // (256 x 256 with a K-contiguous B runs the split-ring engine instead;
     // with an M/N-contiguous B, 64-byte rows and four stages measured best)
-    static constexpr int RB = RBX ? RBX : ((MT_M == 64 && MT_N == 64 && AK && BK) ? 128 : 64);  // LDS bytes per row per stage
+    static constexpr int RB = RBX ? RBX : (((MT_M == 64 && MT_N == 64) || (MT_M == 128 && MT_N == 128)) && AK && BK ? 128 : 64);  // LDS bytes per row per stage
     static constexpr int DU = RB / 2;                  // K per stage (16-bit)
     static constexpr int RS = DU / 16;                 // MFMA read steps per stage
     // two workgroups per CU where the registers allow it without spills: the
@@ -48,7 +50,7 @@ struct Cfg {
     // M/N-contiguous operand is the 128-row side
     static constexpr bool kTwoFit = kSmall || (AK && BK) || (MT_M == 128 && MT_N == 256 && !AK && BK) ||
                                     (MT_M == 256 && MT_N == 128 && AK && !BK);
-    static constexpr int OCC = kBig ? 1 : (kTwoFit ? 2 : 1);
+    static constexpr int OCC = kBig ? 1 : ((kTwoFit && !(kSmall && RB == 128)) ? 2 : 1);
     // four stages (three K-tiles in flight), except the two-per-CU 128 x 256 /
     // 256 x 128 shapes; 64 x 64, whose small problems are a latency chain of
     // K-tiles, measured 7-21 % faster with four than with three

@@ -461,12 +461,14 @@ def test_tile_from_reference_exact_and_nearest():
     # 256 x 256 with 2 x 4 waves: the eight-wave engine, exactly
     st, t, _ = tl.tile_from_reference(p, 4, 2, 2, 4, 32)
     assert st == C.SUCCESS and (t.mt_m, t.mt_n, t.waves_m, t.waves_n) == (256, 256, 2, 4)
-    # 128 x 128 with 2 x 2 waves and DepthU 32 (the pipelined engine's 64-byte K rows): exact
-    st, t, _ = tl.tile_from_reference(p, 2, 2, 2, 2, 32, 32)
-    assert st == C.SUCCESS and (t.mt_m, t.mt_n, t.depth_u) == (128, 128, 32)
-    # ... with DepthU 16: the same tile, flagged
-    st, t, msg = tl.tile_from_reference(p, 2, 2, 2, 2, 32, 16)
-    assert st == C.WARN and (t.mt_m, t.mt_n) == (128, 128) and "depth_u 16" in msg
+    # 128 x 128 with 2 x 2 waves and DepthU 64 (the pipelined engine's 128-byte K rows, both operands
+    # K-contiguous): exact
+    st, t, _ = tl.tile_from_reference(p, 2, 2, 2, 2, 32, 64)
+    assert st == C.SUCCESS and (t.mt_m, t.mt_n, t.depth_u) == (128, 128, 64)
+    # ... with DepthU 32 or 16: the same tile, flagged
+    for du in (32, 16):
+        st, t, msg = tl.tile_from_reference(p, 2, 2, 2, 2, 32, du)
+        assert st == C.WARN and (t.mt_m, t.mt_n, t.depth_u) == (128, 128, 64) and f"depth_u {du}" in msg
     # 128 x 64 has no kernel: the smallest covering tile, 128 x 128
     st, t, msg = tl.tile_from_reference(p, 2, 1, 2, 2, 32)
     assert st == C.WARN and (t.mt_m, t.mt_n) == (128, 128) and "MT128x64" in msg
