@@ -29,20 +29,22 @@ namespace v2 {
 
 // Per-tile pipeline configuration: 64-byte rows (K = 32 per stage), 3-4
 // stages; two workgroups per CU where the registers allow, so one group's
-// MFMAs cover the other's waits.  64 x 64 and 128 x 128 with both operands
-// K-contiguous take 128-byte rows (K = 64 per stage): twice the K in flight
-// and half the barriers.  Measured on MI355X: 64 x 64 fp16 / bf16 4096 x 512 x
-// 4096 34.9 -> 28.3 us, the one-round shapes of 512 x 512 x 512 to 512 x 2048
-// x 2048 within 2 %; 128 x 128, now one workgroup per CU (128 KiB of LDS),
-// 1-4 % faster from 2048^3 to 4096^3 and on 480 x 8192 x 8192.  RBX > 0 sets
-// the row bytes instead.
+// MFMAs cover the other's waits.  Tiles with both operands K-contiguous
+// (TN) take 128-byte rows (K = 64 per stage): twice the K in flight and half
+// the barriers.  Measured on MI355X: 64 x 64 fp16 / bf16 4096 x 512 x 4096
+// 34.9 -> 28.3 us, the one-round shapes of 512 x 512 x 512 to 512 x 2048 x
+// 2048 within 2 %; 128 x 128, now one workgroup per CU (128 KiB of LDS), 1-4 %
+// faster from 2048^3 to 4096^3 and on 480 x 8192 x 8192; 256 x 128 and 128 x
+// 256, now one workgroup per CU with three stages (144 KiB), 2-6 % faster
+// (fp16 8192 x 1024 x 8192 143.6 -> 135.7 us).  RBX > 0 sets the row bytes
+// instead.
 template <int MT_M, int MT_N, bool AK, bool BK, int RBX = 0>
 struct Cfg {
     static constexpr bool kBig = (MT_M == 256 && MT_N == 256);
     static constexpr bool kSmall = (MT_M == 128 && MT_N == 128);
     // (256 x 256 with a K-contiguous B runs the split-ring engine instead;
     // with an M/N-contiguous B, 64-byte rows and four stages measured best)
-    static constexpr int RB = RBX ? RBX : (((MT_M == 64 && MT_N == 64) || (MT_M == 128 && MT_N == 128)) && AK && BK ? 128 : 64);  // LDS bytes per row per stage
+    static constexpr int RB = RBX ? RBX : (!kBig && AK && BK ? 128 : 64);  // LDS bytes per row per stage
     static constexpr int DU = RB / 2;                  // K per stage (16-bit)
     static constexpr int RS = DU / 16;                 // MFMA read steps per stage
     // two workgroups per CU where the registers allow it without spills: the
@@ -50,11 +52,13 @@ struct Cfg {
     // M/N-contiguous operand is the 128-row side
     static constexpr bool kTwoFit = kSmall || (AK && BK) || (MT_M == 128 && MT_N == 256 && !AK && BK) ||
                                     (MT_M == 256 && MT_N == 128 && AK && !BK);
-    static constexpr int OCC = kBig ? 1 : ((kTwoFit && !(kSmall && RB == 128)) ? 2 : 1);
+    // (one, with 128-byte rows above 64 x 64: their stages fill most of the LDS)
+    static constexpr int OCC = kBig ? 1 : ((kTwoFit && !(MT_M + MT_N > 128 && RB == 128)) ? 2 : 1);
     // four stages (three K-tiles in flight), except the two-per-CU 128 x 256 /
-    // 256 x 128 shapes; 64 x 64, whose small problems are a latency chain of
-    // K-tiles, measured 7-21 % faster with four than with three
-    static constexpr int NS = kBig ? 4 : ((OCC == 2 && !kSmall && MT_M > 64) ? 3 : 4);
+    // 256 x 128 shapes and those shapes with 128-byte rows (four would exceed
+    // the LDS); 64 x 64, whose small problems are a latency chain of K-tiles,
+    // measured 7-21 % faster with four than with three
+    static constexpr int NS = kBig ? 4 : (((OCC == 2 && !kSmall && MT_M > 64) || (RB == 128 && MT_M + MT_N > 256)) ? 3 : 4);
     static constexpr int PF = NS - 1;                  // K-tiles in flight
 };
 
