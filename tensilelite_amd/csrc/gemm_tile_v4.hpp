@@ -274,6 +274,79 @@ __device__ inline void store_tile_staged(const TileArgs& a, char* Dt, const f32x
     }
 }
 
+// beta != 0 with 16-bit D and C: the unscaled accumulators are staged in fp32,
+// two rounds of 128 rows (group pg's half in round pg, as for fp32 D).  Each
+// round's C pieces (4 consecutive rows of one column per thread, 8 bytes) are
+// loaded before the round's staging barrier, so their latency overlaps the
+// staging; the column pass then forms fma(alpha, acc, beta * C), the
+// per-fragment epilogue's arithmetic, and stores whole 256-byte column pieces
+// of D.  Dt, Ct: the tile's first element of D and of C.
+template <int OUT>
+__device__ inline void store_tile_staged_beta(const TileArgs& a, char* Dt, const char* Ct,
+                                              const f32x16_t (&acc)[TI][TJ], float alpha, char* smem, int tid) {
+    static_assert(OUT == kOutBF16 || OUT == kOutF16, "16-bit D and C");
+    typedef int v2i_t __attribute__((ext_vector_type(2)));
+    constexpr int CS = 528;   // LDS bytes per staged column (512 + 16)
+    constexpr int ROWS = 128; // rows per staging round
+    constexpr int NT = 2;     // buffer store cache policy: nt
+    const int lane = tid & 63;
+    const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
+    const int grp = wave >> 2, s = wave & 3;
+    const int h = lane >> 5, ch = tid & 31;
+    const bool trunc = OUT == kOutBF16 && (a.flags & kFlagTrunc) != 0;
+    const float beta = a.beta;
+    const __amdgpu_buffer_rsrc_t rd = make_rsrc(Dt, (uint32_t)((int64_t)(MT - 1) * a.ldd * 2 + MT * 2));
+    const __amdgpu_buffer_rsrc_t rc = make_rsrc(Ct, (uint32_t)((int64_t)(MT - 1) * a.ldc * 2 + MT * 2));
+    // the groups realign (group 1 ran one barrier ahead), then every wave's
+    // loads have landed and the stages are free
+    if (grp == 0) barrier();
+    barrier();
+#pragma unroll
+    for (int pg = 0; pg < 2; ++pg) {
+        // pass p of this round: column 16 p + tid / 32, rows 128 pg + 4 ch .. + 3
+        v2i_t c[16];
+#pragma unroll
+        for (int pass = 0; pass < 16; ++pass) {
+            const int col = 16 * pass + (tid >> 5);
+            c[pass] = __builtin_amdgcn_raw_buffer_load_b64(
+                rc, (uint32_t)(((int64_t)col * a.ldc + ROWS * pg + 4 * ch) * 2), 0, 0);
+        }
+        if (grp == pg) {
+#pragma unroll
+            for (int i = 0; i < TI; ++i)
+#pragma unroll
+                for (int j = 0; j < TJ; ++j) {
+                    const f32x16_t& x = acc[i][j];
+                    char* p = smem + (64 * s + 32 * j + (lane & 31)) * CS + (32 * i + 16 * h) * 4;
+#pragma unroll
+                    for (int q = 0; q < 4; ++q)
+                        *(f32x4_t*)(p + 16 * q) = f32x4_t{x[4 * q], x[4 * q + 1], x[4 * q + 2], x[4 * q + 3]};
+                }
+        }
+        lds_barrier();
+#pragma unroll
+        for (int pass = 0; pass < 16; ++pass) {
+            const int col = 16 * pass + (tid >> 5);
+            const f32x4_t v = *(const f32x4_t*)(smem + col * CS + ch * 16);
+            const uint32_t u0 = (uint32_t)c[pass][0], u1 = (uint32_t)c[pass][1];
+            const float cv[4] = {OUT == kOutBF16 ? bf16lo(u0) : f16lo(u0), OUT == kOutBF16 ? bf16hi(u0) : f16hi(u0),
+                                 OUT == kOutBF16 ? bf16lo(u1) : f16lo(u1), OUT == kOutBF16 ? bf16hi(u1) : f16hi(u1)};
+            float r[4];
+#pragma unroll
+            for (int e = 0; e < 4; ++e) r[e] = __builtin_fmaf(alpha, v[e], beta * cv[e]);
+            uint32_t w[2];
+#pragma unroll
+            for (int e = 0; e < 2; ++e)
+                w[e] = OUT == kOutF16 ? pack_f16x2(r[2 * e], r[2 * e + 1])
+                                      : (trunc ? pack_bf16x2_trunc(r[2 * e], r[2 * e + 1])
+                                               : pack_bf16x2_rne(r[2 * e], r[2 * e + 1]));
+            __builtin_amdgcn_raw_buffer_store_b64(v2i_t{(int)w[0], (int)w[1]}, rd,
+                                                  (uint32_t)(((int64_t)col * a.ldd + ROWS * pg + 4 * ch) * 2), 0, NT);
+        }
+        if (pg == 0) lds_barrier();
+    }
+}
+
 template <int MODE, bool AK, bool BK, int OUT>
 __device__ inline void gemm_tile(const TileArgs& a, int tm, int tn, int b, char* smem, int tid) {
     static_assert(MODE == kBF16 || MODE == kF16 || MODE == kF8, "16-bit or fp8 engine");
@@ -497,14 +570,23 @@ __device__ inline void gemm_tile(const TileArgs& a, int tm, int tn, int b, char*
 
     const char* Cb = (const char*)a.C + (int64_t)b * a.bsC * ((OUT == kOutF32) ? 4 : 2);
     char* Db = (char*)a.D + (int64_t)b * a.bsD * ((OUT == kOutF32) ? 4 : 2);
-    // A whole tile with beta = 0 leaves through LDS as whole columns (see
-    // store_tile_staged); edge tiles, beta != 0 and D extents past the 31-bit
-    // buffer range store fragment by fragment.
+    // A whole tile leaves through LDS as whole columns (store_tile_staged; with
+    // beta != 0 and 16-bit D, store_tile_staged_beta); edge tiles, fp32 D with
+    // beta != 0 and C or D extents past the 31-bit buffer range store fragment
+    // by fragment.
     constexpr int OS = (OUT == kOutF32) ? 4 : 2;
-    if (m0 + MT <= a.M && n0 + MT <= a.N && (a.flags & kFlagBeta) == 0 &&
-        (int64_t)(MT - 1) * a.ldd * OS + MT * OS < 0x7ffffff0ll) {
-        store_tile_staged<OUT>(a, Db + ((int64_t)n0 * a.ldd + m0) * OS, acc, alpha, smem, tid);
-        return;
+    if (m0 + MT <= a.M && n0 + MT <= a.N && (int64_t)(MT - 1) * a.ldd * OS + MT * OS < 0x7ffffff0ll) {
+        if ((a.flags & kFlagBeta) == 0) {
+            store_tile_staged<OUT>(a, Db + ((int64_t)n0 * a.ldd + m0) * OS, acc, alpha, smem, tid);
+            return;
+        }
+        if constexpr (OUT != kOutF32) {
+            if ((int64_t)(MT - 1) * a.ldc * OS + MT * OS < 0x7ffffff0ll) {
+                store_tile_staged_beta<OUT>(a, Db + ((int64_t)n0 * a.ldd + m0) * OS, Cb + ((int64_t)n0 * a.ldc + m0) * OS,
+                                            acc, alpha, smem, tid);
+                return;
+            }
+        }
     }
     int fm[TI * TJ], fn[TI * TJ];
 #pragma unroll
