@@ -460,6 +460,40 @@ def test_bf16_truncate_matches_reference_store(cuda):
     np.testing.assert_array_equal(got, want)
 
 
+@pytest.mark.parametrize("layout", ["TN", "NT"])
+@pytest.mark.parametrize("name,rounding", [("bf16", "rne"), ("bf16", "truncate"), ("f16", "rne")])
+def test_staged_beta_epilogue_bit_exact(cuda, layout, name, rounding):
+    """beta != 0 on the eight-wave engine: whole 256 x 256 tiles leave through
+    LDS (fp32 staging, C read as 8-byte column pieces), the edge tiles of the
+    same launch through the per-fragment epilogue.  C and D have different
+    padded leading dimensions.  Integer data and power-of-two alpha / beta keep
+    every fp32 result exact, so D must equal the oracle's conversion of the
+    exact result to the bit, under either rounding."""
+    M, N, K = 640, 512, 192          # M: two whole tile rows and one of 128 rows
+    ldc, ldd = M + 24, M + 8
+    _say(f"case staged beta {layout} {name} {rounding}")
+    rng = np.random.default_rng(17)
+    (ra, ca), (rb, cb) = op_dims(layout, M, N, K)
+    A = colmajor(rng, ra, ca, kind="int")
+    B = colmajor(rng, rb, cb, kind="int")
+    Cm = colmajor(rng, M, N, ldc, kind="int")
+    Ad, Bd, Cd = (_dev16(quantize_bits(x, name), name, cuda) for x in (A, B, Cm))
+    Dd = torch.full((ldd * N,), float("nan"), dtype=Ad.dtype, device=cuda)
+    alpha, beta = 0.5, -0.25
+    rnd = C.ROUND_TRUNCATE if rounding == "truncate" else C.ROUND_RNE
+    # the eight-wave engine at 256 x 256, no repack
+    tl.gemm(layout, M, N, K, Ad, Bd, Cd, Dd, ldc=ldc, ldd=ldd, alpha=alpha, beta=beta, round=rnd,
+            tile=(256, 256, 0, 2, 4, 0, 0, 1))
+    torch.cuda.synchronize()
+    exact = oracle.gemm_f64(layout, M, N, K, alpha, A.astype(np.float64), ra, B.astype(np.float64), rb, beta,
+                            Cm.astype(np.float64), ldc, ldd).astype(np.float32)
+    got = Dd.view(torch.int16).cpu().numpy().view(np.uint16).reshape(N, ldd)
+    want = quantize_bits(np.ascontiguousarray(extract(exact, M, N, ldd)[0]).ravel(), name,
+                         truncate=rounding == "truncate").reshape(N, M)
+    np.testing.assert_array_equal(got[:, :M], want)
+    assert np.isnan(Dd.float().cpu().numpy().reshape(N, ldd)[:, M:]).all()   # padding rows untouched
+
+
 def test_fp8_quantize_bit_exact_and_gemm(cuda):
     M, N, K = 192, 160, 256
     _say("case fp8")
