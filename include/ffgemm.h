@@ -155,7 +155,9 @@ int ffgemm_tune_db_load(const char *path, int *entries);
 /* D = alpha*op(A)*op(B) + beta*C on `stream` (hipStream_t, NULL = null stream).
  * Replaces the path GemmMfmaAsmSolution::generateKernel -> StreamHip::Launch
  * (src/gemmMfma.cpp:24-185, ff_gpu_src/ff_hip_runtime.cpp:278-320).
- * `tile` may be NULL.  beta == 0 never reads C.
+ * `tile` may be NULL.  beta == 0 never reads C.  C may be the same buffer as
+ * D when ldc == ldd and bsC == bsD (an in-place update: each element of C is
+ * read by the thread that later writes it); other overlaps are not supported.
  * Leading dimensions and batch strides of any length, any K, and pointers
  * aligned to their element size are accepted.  Problems with every ld, batch
  * stride and pointer on the 16-byte grid (and K a multiple of 16 bytes of

@@ -884,3 +884,48 @@ def test_16x16_mfma_form_f32_out(cuda, layout, mt):
     torch.cuda.synchronize()
     ref = _f64_ref(layout, M, N, K, bits_to_f32(Ab, "bf16"), ra, bits_to_f32(Bb, "bf16"), rb, 2.0, 0.25, Cm)
     assert rel_fro(D.cpu().numpy(), ref) <= 1e-5
+
+
+# (dtype, layout, M, N, K, tile record or None)
+INPLACE_CASES = [
+    ("bf16", "TN", 512, 512, 128, (256, 256, 0, 2, 4, 0, 0, 1)),   # eight-wave engine, staged beta epilogue
+    ("bf16", "TN", 520, 264, 128, (256, 256, 0, 2, 4, 0, 0, 1)),   # eight-wave engine, edge tiles
+    ("f16", "NN", 520, 264, 96, (256, 256, 0, 2, 2, 0, 0, 1)),     # split-ring engine, C requested first
+    ("f16", "TN", 300, 200, 136, None),                            # the library's tile
+    ("bf16", "NT", 264, 392, 512, (128, 128, 0, 0, 0, 0, 2)),      # split-K into the workspace, then the reduce
+    ("bf16", "NN", 384, 256, 200, (0, 0, 0, 0, 0, 0, 0, 2)),       # K-major repack, then TN
+    ("f32", "NN", 200, 136, 64, None),                             # fp32
+]
+
+
+@pytest.mark.parametrize("case", INPLACE_CASES, ids=lambda c: "_".join(map(str, c[:5])))
+def test_in_place_beta_update(cuda, case):
+    """D = alpha*op(A)*op(B) + beta*C with D the same buffer as C (ldc = ldd):
+    every epilogue reads each C element in the thread that later writes that
+    D element, so the in-place result equals the out-of-place one bit for bit."""
+    name, layout, M, N, K, tile = case
+    _say(f"case in-place {name} {layout} {M}x{N}x{K} tile={tile}")
+    rng = np.random.default_rng(M + N + K)
+    (ra, ca), (rb, cb) = op_dims(layout, M, N, K)
+    ld = (M + 7) // 8 * 8 + 8
+    A = colmajor(rng, ra, ca, kind="normal", scale=1 / np.sqrt(K))
+    B = colmajor(rng, rb, cb, kind="normal", scale=1 / np.sqrt(K))
+    Cm = colmajor(rng, M, N, ld, kind="normal")
+    if name == "f32":
+        Ad, Bd, Cd = (torch.from_numpy(x).to(cuda) for x in (A, B, Cm))
+        code = C.F32
+    else:
+        Ad, Bd, Cd = (_dev16(quantize_bits(x, name), name, cuda) for x in (A, B, Cm))
+        code = C.BF16 if name == "bf16" else C.F16
+    p = tl.make_problem(layout, code, M, N, K, ra, rb, ld, ld, 0.625, -1.25)
+    D = torch.full_like(Cd, float("nan"))
+    tl.run(p, Ad, Bd, Cd, D, tile=tile)
+    inplace = Cd.clone()
+    tl.run(p, Ad, Bd, inplace, inplace, tile=tile)
+    torch.cuda.synchronize()
+    got = inplace.cpu().view(torch.int32 if name == "f32" else torch.int16).numpy().reshape(N, ld)
+    want = D.cpu().view(torch.int32 if name == "f32" else torch.int16).numpy().reshape(N, ld)
+    np.testing.assert_array_equal(got[:, :M], want[:, :M])
+    # the padding rows keep C's values
+    cref = Cd.cpu().view(torch.int32 if name == "f32" else torch.int16).numpy().reshape(N, ld)
+    np.testing.assert_array_equal(got[:, M:], cref[:, M:])
